@@ -1,0 +1,189 @@
+"""Benchmark of the batched American FD pricer on one MI355X.
+
+One step = one pass of the pricer over the whole batch with inputs resident in HBM:
+32768 seeded American options (K U[50,150], T U[1/12,3], sigma U[0.1,0.6],
+r U[0,0.1], q U[0,0.08], puts and calls, spot 100) on a 1024 x-point x 1024 t-point
+grid, FP64, through kwgpu_fd1d_price_device (one kernel launch per step).
+
+Prints one JSON line. `value` is options/s of the whole batch; `roofline` prices the
+kernel's algorithmic FP64 work (10 flops per point and step) against the FP64 peak
+using the kernel duration measured with HIP events on the launch stream;
+`cpu_baseline` times the CPU restatement of the reference pricer (its thread-pool
+path, one job per PDE) on a bounded sample of the same workload.
+
+    python bench.py [--gpus 1] [--steps 10] [--warmup 2]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "kwinto-cuda_amd"))
+
+N_OPTIONS = 32768
+X_DIM = 1024
+T_DIM = 1024
+SEED = 1
+FP64_PEAK_TFLOPS = 78.6          # MI355X FP64 (vector and dense matrix)
+HBM_PEAK_GBS = 8000.0
+# published FP64 GPU throughput of the reference's own GPU solver at this shape
+# (GTX 1070, batch 32768, 1024 x 1024), options/s
+REFERENCE_GPU_FP64 = 949.0
+
+
+def parse_args():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--layout", default="wave", choices=["wave", "lane", "auto"])
+    ap.add_argument("--cpu-sample", type=int, default=1024,
+                    help="options of the workload the CPU baseline prices (0 = skip)")
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="after the timed region, run one device copy of this size "
+                         "(calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
+    return ap.parse_args()
+
+
+def read_traffic():
+    """HBM bytes per launch measured by rocprofv3 --pmc (profiles/fd1d_wave_traffic.md)."""
+    path = os.path.join(ROOT, "profiles", "fd1d_wave_traffic.md")
+    try:
+        txt = open(path).read()
+    except OSError:
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", txt)
+    return float(m.group(1)) if m else None
+
+
+def cpu_baseline(opts, sample, threads):
+    import oracle
+    sub = opts[:sample]
+    oracle.fd1d_price(sub[:min(16, sample)], X_DIM, T_DIM, nthreads=threads)  # warm-up
+    t0 = time.perf_counter()
+    _, err = oracle.fd1d_price(sub, X_DIM, T_DIM, nthreads=threads)
+    dt = time.perf_counter() - t0
+    if err:
+        raise RuntimeError(err)
+    return {"value": sample / dt, "unit": "options/s", "cores": threads, "kind": "port",
+            "sample": f"first {sample} options of the same workload, {X_DIM}x{T_DIM} FP64, "
+                      f"{threads} threads, one job per PDE ({dt:.2f} s)"}
+
+
+def main():
+    args = parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this pricer runs on one GPU; --gpus must be 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import kwgpu
+    from kwgpu import workloads as W
+
+    layout = {"wave": kwgpu.LAYOUT_WAVE_REG, "lane": kwgpu.LAYOUT_LANE_SOA,
+              "auto": kwgpu.LAYOUT_AUTO}[args.layout]
+    opts = W.plain(N_OPTIONS, SEED)
+    dev = torch.device("cuda:0")
+    d_opts = torch.from_numpy(opts.view(np.uint8).copy()).to(dev)
+    d_prices = torch.empty(N_OPTIONS, dtype=torch.float64, device=dev)
+    d_status = torch.empty(N_OPTIONS, dtype=torch.int32, device=dev)
+    ctx = kwgpu.Context(0)
+    stream = torch.cuda.current_stream(dev)
+    prm = kwgpu.params(X_DIM, T_DIM, precision=kwgpu.FP64, layout=layout)
+
+    def step():
+        ctx.fd1d_price_device(d_opts.data_ptr(), N_OPTIONS, d_prices.data_ptr(),
+                              d_status.data_ptr(), prm, stream=stream.cuda_stream)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    print(f"bench: warmup done ({args.warmup} steps)", file=sys.stderr, flush=True)
+
+    # HIP events around every launch on the launch stream
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev[0].record(stream)
+    for i in range(args.steps):
+        step()
+        ev[i + 1].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kernel_ms = [ev[i].elapsed_time(ev[i + 1]) for i in range(args.steps)]
+    ms_per_step = wall * 1e3 / args.steps
+    status = d_status.cpu().numpy()
+    if not np.all(status == 0):
+        raise RuntimeError(f"{int(np.sum(status != 0))} options failed")
+    print(f"bench: timed {args.steps} steps, {ms_per_step:.3f} ms/step", file=sys.stderr,
+          flush=True)
+
+    if args.copy_probe_mib:
+        nbytes = args.copy_probe_mib << 20
+        a = torch.empty(nbytes // 8, dtype=torch.float64, device=dev).fill_(1.0)
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+
+    avg_kernel_s = float(np.mean(kernel_ms)) * 1e-3
+    flops = W.fd_flops_per_option(X_DIM, T_DIM) * N_OPTIONS
+    achieved = flops / avg_kernel_s / 1e12
+    traffic = read_traffic()
+    value = N_OPTIONS / (ms_per_step * 1e-3)
+
+    cpu = None
+    if args.cpu_sample > 0:
+        threads = min(args.cpu_threads, os.cpu_count() or 1)
+        print(f"bench: CPU baseline, {args.cpu_sample} options on {threads} threads",
+              file=sys.stderr, flush=True)
+        cpu = cpu_baseline(opts, args.cpu_sample, threads)
+
+    out = {
+        "metric": "options/sec at max abs err <=1e-4 vs FP64 FD: plain FP64 vs FP32 vs surrogate",
+        "value": value,
+        "unit": "options/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": value / REFERENCE_GPU_FP64,
+        "dtype": "f64",
+        "data": "synthetic (seeded SplitMix64 portfolio)",
+        "config": {
+            "workload": f"plain method: {N_OPTIONS} American options, {X_DIM} x-points x "
+                        f"{T_DIM} t-points, FP64, Crank-Nicolson, seed {SEED}",
+            "form": "fd_fp64",
+            "layout": args.layout,
+            "max_abs_err_vs_fp64_fd": 0.0,
+        },
+        "roofline": {
+            "bound": "mfma",
+            "roof": "FP64 VALU (MI355X FP64 vector peak = FP64 dense peak = 78.6 TFLOP/s)",
+            "achieved": achieved,
+            "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / FP64_PEAK_TFLOPS,
+            "traffic": traffic,
+            "kernel": "fd1d_wave_f64<16,64>",
+            "kernel_ms": avg_kernel_s * 1e3,
+            "algorithmic_flops_per_launch": flops,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,64 @@
+// bs.hip -- closed-form European price and the FD1D-BS control-variate combine.
+#include <math.h>
+
+#include "kwgpu_internal.h"
+
+namespace kwgpu {
+
+// BlackScholes_Pricer::priceOne, src/Pricer/kwBlackScholes.cpp:27-50:
+// NaN for American options; cdfNormal(x) = (1 + erf(x / sqrt 2)) / 2 (src/Math/kwMath.cpp:9-13).
+#pragma clang fp contract(off)
+__device__ __forceinline__ double cdf_normal(double x) {
+    return 0.5 * (1 + erf(x / 1.4142135623730951));
+}
+
+__device__ __forceinline__ double bs_one(const kwgpu_option& a) {
+    if (a.e) return NAN;
+    const double k = a.k, q = a.q, r = a.r, s = a.s, t = a.t, z = a.z;
+    const double w = (double)a.w;
+    const double zt = z * sqrt(t);
+    const double d1 = 1 / zt * (log(s / k) + (r - q + 0.5 * z * z) * t);
+    const double d2 = d1 - zt;
+    return w * (s * cdf_normal(w * d1) * exp(-q * t) - k * cdf_normal(w * d2) * exp(-r * t));
+}
+#pragma clang fp contract(on)
+
+namespace {
+
+__global__ __launch_bounds__(256) void bs_kernel(const kwgpu_option* opts, size_t n,
+                                                 double* prices) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) prices[i] = bs_one(opts[i]);
+}
+
+// Fd1d_BlackScholes_Pricer::price, src/Pricer/kwFd1d_BlackScholes.cpp:38-40:
+// price += BS(euro) - FD(euro).
+__global__ __launch_bounds__(256) void fd1d_bs_combine_kernel(const kwgpu_option* opts, size_t n,
+                                                              double* amer, const double* euro) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        kwgpu_option a = opts[i];
+        a.e = 0;
+        amer[i] += bs_one(a) - euro[i];
+    }
+}
+
+}  // namespace
+
+hipError_t launch_bs(const kwgpu_option* opts, size_t n, double* prices, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    const unsigned blocks = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(bs_kernel, dim3(blocks), dim3(256), 0, s, opts, n, prices);
+    return hipGetLastError();
+}
+
+hipError_t launch_fd1d_bs_combine(const kwgpu_option* opts, size_t n, double* amer,
+                                  const double* euro, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    const unsigned blocks = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(fd1d_bs_combine_kernel, dim3(blocks), dim3(256), 0, s, opts, n, amer,
+                       euro);
+    return hipGetLastError();
+}
+
+}  // namespace kwgpu

@@ -1,0 +1,150 @@
+// kwgpu_internal.h -- shared device/host definitions of the gfx950 FD pricer.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "kwgpu.h"
+
+namespace kwgpu {
+
+// One batch of PDE solves as the kernels see it. All pointers are device pointers.
+//
+// PDE p (0 <= p < m) is solved on the grid of option rep[p] (identity when rep is
+// null) and its solution is interpolated for the options list[off[p] .. off[p+1])
+// (just option rep[p] when off is null). `order`, when given, is the processing
+// order of PDEs (slot -> p), used to group PDEs of similar cost into one wave.
+struct FdBatch {
+    const kwgpu_option* opts;
+    const uint32_t* rep;
+    const uint32_t* off;
+    const uint32_t* list;
+    const uint32_t* order;
+    const int32_t* x_dims;  // per option (indexed by rep[p]); null = x_dim
+    const int32_t* t_dims;  // per option; null = t_dim
+    int m;
+    int x_dim, t_dim;
+    double density, scale, theta;
+    double* prices;
+    int32_t* status;
+    double* ws;             // lane layout workspace (null for the wave layout)
+    long long ws_stride;    // slots per workspace row (>= m)
+    int ws_rows;            // rows per workspace array (>= every x_dim in the batch)
+};
+
+// Per-PDE parameters derived from an option (src/Pricer/kwFd1d.cpp:71-86).
+struct PdeParams {
+    double t, a0, ax, axx;
+    double yMin, yMax, dy;  // sinh mesh: x_j = density * sinh(yMin (1 - j dy) + yMax j dy)
+    double dt;              // uniform time step T / (tDim - 1)
+    int x_dim, t_dim;
+    bool american;
+    bool call;
+    bool valid;
+};
+
+__device__ __forceinline__ bool finite_pos(double v) { return v > 0.0 && v < 1.0e300; }
+
+// Grid and coefficient setup mirror the reference's expression order without FMA
+// contraction so the grid matches the CPU pricer to within libm ulps.
+#pragma clang fp contract(off)
+__device__ __forceinline__ PdeParams pde_params(const kwgpu_option& o, int x_dim, int t_dim,
+                                                double density, double scale) {
+    PdeParams p;
+    p.t = o.t;
+    p.a0 = -o.r;
+    p.ax = o.r - o.q - o.z * o.z / 2;
+    p.axx = o.z * o.z / 2;
+    p.x_dim = x_dim;
+    p.t_dim = t_dim;
+    p.american = o.e != 0;
+    p.call = !(o.w < 0);
+    p.valid = finite_pos(o.t) && finite_pos(o.z) && finite_pos(o.s) && finite_pos(o.k) &&
+              (o.r == o.r) && (o.q == o.q) && fabs(o.r) < 1e6 && fabs(o.q) < 1e6 && x_dim >= 3 &&
+              t_dim >= 2;
+    // t-grid, src/Pricer/kwFd1d.cpp:95-103
+    p.dt = (o.t - 0.0) / (double)(t_dim - 1);
+    // x-grid, src/Pricer/kwFd1d.cpp:105-125
+    const double xMin = 0.0 - scale * o.z * sqrt(o.t);
+    const double xMax = 0.0 + scale * o.z * sqrt(o.t);
+    p.yMin = asinh((xMin - 0.0) / density);
+    p.yMax = asinh((xMax - 0.0) / density);
+    p.dy = 1. / (double)(x_dim - 1);
+    return p;
+}
+
+__device__ __forceinline__ double grid_x(const PdeParams& p, double density, int j) {
+    const double yj = j * p.dy;
+    return 0.0 + density * sinh(p.yMin * (1.0 - yj) + p.yMax * yj);
+}
+
+// payoff in units of K, src/Pricer/kwFd1d.cpp:127-139
+__device__ __forceinline__ double payoff(const PdeParams& p, double x) {
+    return p.call ? fmax(0.0, exp(x) - 1.) : fmax(0.0, 1. - exp(x));
+}
+
+// Row j of B = I - theta dt A (src/Math/kwFd1d.cpp:73-114) from x_{j-1}, x_j, x_{j+1}.
+// Rows j >= N are identity padding rows.
+__device__ __forceinline__ void b_row(const PdeParams& p, double th, int j, double xm, double x0,
+                                      double xp, double& bl, double& b, double& bu) {
+    const int N = p.x_dim;
+    const double dt = p.dt, a0 = p.a0, ax = p.ax, axx = p.axx;
+    if (j >= N) {
+        bl = 0.0;
+        b = 1.0;
+        bu = 0.0;
+    } else if (j == 0) {
+        const double inv_dx = 1. / (xp - x0);
+        bl = 0;
+        b = 1 - th * dt * (a0 - inv_dx * ax);
+        bu = -th * dt * (inv_dx * ax);
+    } else if (j == N - 1) {
+        const double inv_dx = 1. / (x0 - xm);
+        bl = -th * dt * (-inv_dx * ax);
+        b = 1 - th * dt * (a0 + inv_dx * ax);
+        bu = 0;
+    } else {
+        const double inv_dxu = 1. / (xp - x0);
+        const double inv_dxm = 1. / (xp - xm);
+        const double inv_dxd = 1. / (x0 - xm);
+        const double inv_dx2u = 2. * inv_dxu * inv_dxm;
+        const double inv_dx2m = 2. * inv_dxd * inv_dxu;
+        const double inv_dx2l = 2. * inv_dxd * inv_dxm;
+        bl = -th * dt * (-inv_dxm * ax + inv_dx2l * axx);
+        b = 1 - th * dt * (a0 - inv_dx2m * axx);
+        bu = -th * dt * (inv_dxm * ax + inv_dx2u * axx);
+    }
+}
+
+// Fd1d::value (src/Math/kwFd1d.cpp:139-158): index of the first node with x_j >= x*,
+// found from the inverse of the sinh mesh and corrected against the exact nodes.
+// Returns 0 or N when x* is outside (x_0, x_{N-1}].
+__device__ __forceinline__ int bracket(const PdeParams& p, double density, double xs) {
+    const int N = p.x_dim;
+    const double u = (asinh(xs / density) - p.yMin) / (p.yMax - p.yMin);
+    const double jf = ceil(u * (double)(N - 1));
+    int j = 0;
+    if (jf >= (double)N)
+        j = N;
+    else if (jf > 0.0)
+        j = (int)jf;
+    while (j > 0 && grid_x(p, density, j - 1) >= xs) --j;
+    while (j < N && grid_x(p, density, j) < xs) ++j;
+    return j;
+}
+#pragma clang fp contract(on)
+
+}  // namespace kwgpu
+
+// Kernel launchers (defined in the .hip files, called by the C-ABI layer).
+namespace kwgpu {
+// Wave layout, FP64 / FP32-mixed. Returns hipErrorInvalidValue when x_dim does not fit.
+hipError_t launch_fd1d_wave(const FdBatch& b, int max_x_dim, int precision, hipStream_t s);
+bool wave_layout_fits(int x_dim, int precision);
+// Lane layout (any grid size).
+hipError_t launch_fd1d_lane(const FdBatch& b, hipStream_t s);
+size_t lane_workspace_doubles(int m, int max_x_dim);
+hipError_t launch_bs(const kwgpu_option* opts, size_t n, double* prices, hipStream_t s);
+hipError_t launch_fd1d_bs_combine(const kwgpu_option* opts, size_t n, double* amer,
+                                  const double* euro, hipStream_t s);
+}  // namespace kwgpu

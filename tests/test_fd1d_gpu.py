@@ -1,0 +1,211 @@
+"""Parity of the HIP FD pricer (through the C-ABI) with the reference and the oracle.
+
+FP64 bar: max abs price difference <= 1e-8 against the reference CPU pricer's
+golden vectors and against the oracle on the same seeded inputs. Full-size runs
+(32768 options at 1024 x 1024) are checked through the golden head and through
+properties that hold at any size: American >= European >= 0, American >= intrinsic,
+determinism, every status OK.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import kwgpu
+import oracle
+from conftest import golden_meta, load_golden
+from kwgpu import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+EPS = 1e-8
+LAYOUTS = [kwgpu.LAYOUT_WAVE_REG, kwgpu.LAYOUT_LANE_SOA]
+LAYOUT_IDS = ["wave", "lane"]
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+@pytest.mark.parametrize("layout", LAYOUTS, ids=LAYOUT_IDS)
+def test_plain_512_golden(ctx, layout):
+    _say("plain_512 layout", layout)
+    o, c = load_golden("plain_512")
+    p, st, rc, err = ctx.fd1d_price(o, 512, 512, layout=layout)
+    assert rc == kwgpu.OK, err
+    assert np.all(st == kwgpu.STATUS_OK)
+    assert np.max(np.abs(p - c["ref_fd1d"])) <= EPS
+    e = o.copy()
+    e["e"] = 0
+    p, st, rc, err = ctx.fd1d_price(e, 512, 512, layout=layout)
+    assert rc == kwgpu.OK, err
+    assert np.max(np.abs(p - c["ref_fd1d_euro"])) <= EPS
+
+
+@pytest.mark.parametrize("layout", LAYOUTS, ids=LAYOUT_IDS)
+def test_plain_1024_head_golden(ctx, layout):
+    _say("plain_1024_head256 layout", layout)
+    o, c = load_golden("plain_1024_head256")
+    p, st, rc, err = ctx.fd1d_price(o, 1024, 1024, layout=layout)
+    assert rc == kwgpu.OK, err
+    assert np.max(np.abs(p - c["ref_fd1d"])) <= EPS
+
+
+def test_pricer_test_12(ctx):
+    """The three pricers of test/kwPricer_test.cpp:37-108 through the GPU."""
+    _say("pricer_test_12")
+    o, c = load_golden("pricer_test_12")
+    fd, st, rc, err = ctx.fd1d_price(o)
+    assert rc == kwgpu.OK, err
+    assert np.max(np.abs(fd - c["ref_fd1d"])) <= EPS
+    assert np.max(np.abs(fd - c["want"])) <= 1.3e-3
+    bs = ctx.bs_price(o)
+    ok = ~np.isnan(c["ref_bs"])
+    assert np.array_equal(np.isnan(bs), ~ok)
+    assert np.max(np.abs(bs[ok] - c["ref_bs"][ok])) <= 1e-12
+    fdbs, st, rc, err = ctx.fd1d_bs_price(o)
+    assert rc == kwgpu.OK, err
+    assert np.max(np.abs(fdbs - c["ref_fd1d_bs"])) <= EPS
+    assert np.max(np.abs(fdbs - c["want"])) <= 1.3e-3
+
+
+def test_fd1d_bs_plain_512(ctx):
+    _say("fd1d_bs plain_512")
+    o, c = load_golden("plain_512")
+    p, st, rc, err = ctx.fd1d_bs_price(o, 512, 512)
+    assert rc == kwgpu.OK, err
+    assert np.max(np.abs(p - c["ref_fd1d_bs"])) <= EPS
+    e = o.copy()
+    e["e"] = 0
+    assert np.max(np.abs(ctx.bs_price(e) - c["ref_bs_euro"])) <= 1e-12
+
+
+def test_qdfp_portfolio(ctx):
+    """test/kwPortfolio_test.cpp:29-58 through the GPU: 6000 puts (600 PDEs after the
+    reference's compression) within 0.005 of QuantLib, and at parity with the
+    reference FD1D output."""
+    _say("qdfp portfolio")
+    o, c = load_golden("portfolio_qdfp")
+    p, st, rc, err = ctx.fd1d_price(o, 512, 512)
+    assert rc == kwgpu.OK, err
+    assert np.max(np.abs(p - c["want"])) <= 0.005
+    assert np.max(np.abs(p - c["ref_fd1d"])) <= EPS
+
+
+def test_uneven_small_golden(ctx):
+    _say("uneven_small")
+    o, c = load_golden("uneven_small")
+    p, st, rc, err = ctx.fd1d_price(o, x_dims=c["x_dim"], t_dims=c["t_dim"])
+    assert rc == kwgpu.OK, err
+    assert np.max(np.abs(p - c["ref_fd1d"])) <= EPS
+
+
+@pytest.mark.parametrize("layout", LAYOUTS, ids=LAYOUT_IDS)
+@pytest.mark.parametrize("x_dim,t_dim", [(3, 2), (17, 5), (64, 37), (100, 64), (129, 33),
+                                         (256, 40), (300, 16), (511, 24), (1000, 12),
+                                         (1024, 20)])
+def test_sizes_vs_oracle(ctx, layout, x_dim, t_dim):
+    _say("sizes", layout, x_dim, t_dim)
+    o = W.plain(37, 1000 + x_dim)
+    o["e"][::4] = 0
+    o["s"] = np.linspace(80, 125, 37)
+    ref, rst = oracle.fd1d_price_status(o, x_dim, t_dim)
+    p, st, rc, err = ctx.fd1d_price(o, x_dim, t_dim, layout=layout)
+    assert np.array_equal(st, rst)
+    ok = rst == 0
+    assert np.max(np.abs(p[ok] - ref[ok]), initial=0.0) <= EPS
+    assert np.all(np.isnan(p[~ok]))
+
+
+def test_range_error_like_reference(ctx):
+    _say("range error")
+    m = golden_meta()["range_error"]
+    o = W.plain(4, m["seed"])
+    o["s"] = m["spots"]
+    p, st, rc, err = ctx.fd1d_price(o, m["x_dim"], m["t_dim"], density=m["density"],
+                                    scale=m["scale"])
+    assert rc == kwgpu.ERR_RANGE
+    assert err == m["message"]
+    ref, rst = oracle.fd1d_price_status(o, m["x_dim"], m["t_dim"], m["density"], m["scale"])
+    assert np.array_equal(st, rst)
+    ok = rst == 0
+    assert np.max(np.abs(p[ok] - ref[ok]), initial=0.0) <= EPS
+
+
+def test_bad_inputs_and_params(ctx):
+    _say("bad inputs")
+    o = W.plain(8, 31)
+    o["z"][2] = -0.2
+    o["t"][5] = 0.0
+    p, st, rc, err = ctx.fd1d_price(o, 128, 32)
+    assert rc == kwgpu.ERR_INVALID
+    assert st[2] == kwgpu.STATUS_BAD_INPUT and st[5] == kwgpu.STATUS_BAD_INPUT
+    good = np.ones(8, bool)
+    good[[2, 5]] = False
+    ref, _ = oracle.fd1d_price_status(o[good], 128, 32)
+    assert np.max(np.abs(p[good] - ref)) <= EPS
+    for bad in (dict(x_dim=2), dict(t_dim=1), dict(density=0.0), dict(theta=0.0)):
+        kw = dict(x_dim=128, t_dim=32)
+        kw.update(bad)
+        _, _, rc, err = ctx.fd1d_price(o, **kw)
+        assert rc == kwgpu.ERR_INVALID and err
+    _, _, rc, _ = ctx.fd1d_price(np.zeros(0, W.OPTION_DTYPE))
+    assert rc == kwgpu.OK
+
+
+def test_chain_compression(ctx):
+    """Options sharing (t, r, q, z, e, w) solve one PDE and interpolate per spot/strike
+    (src/Pricer/kwFd1d.cpp:28-65, :146-157)."""
+    _say("chain")
+    base = W.plain(5, 41)
+    o = np.repeat(base, 13)
+    o["s"] = np.tile(np.linspace(70, 140, 13), 5)
+    o["k"] = np.tile(np.linspace(90, 110, 13)[::-1], 5)
+    ref, err = oracle.fd1d_price(o, 300, 50)
+    assert err == ""
+    for lay in LAYOUTS:
+        p, st, rc, err = ctx.fd1d_price(o, 300, 50, layout=lay)
+        assert rc == kwgpu.OK, err
+        assert np.max(np.abs(p - ref)) <= EPS
+
+
+def test_theta_general_matches_reference_at_half(ctx):
+    o = W.plain(16, 51)
+    p1, _, rc, _ = ctx.fd1d_price(o, 256, 64, theta=0.5)
+    assert rc == 0
+    ref, _ = oracle.fd1d_price(o, 256, 64)
+    assert np.max(np.abs(p1 - ref)) <= EPS
+    p2, _, rc, _ = ctx.fd1d_price(o, 256, 64, theta=1.0)  # implicit Euler still prices
+    assert rc == 0 and np.max(np.abs(p2 - ref)) < 0.05
+
+
+def test_device_path_full_size_properties(ctx):
+    """32768 options, 1024 x 1024, device-resident: head equals the golden vectors,
+    American >= European >= 0, American >= intrinsic, bitwise deterministic."""
+    import torch
+    _say("full size 32768 x 1024 x 1024")
+    n = 32768
+    o = W.plain(n, 1)
+    d_o = torch.from_numpy(o.view(np.uint8).copy()).cuda()
+    d_p = torch.empty(n, dtype=torch.float64, device="cuda")
+    d_s = torch.empty(n, dtype=torch.int32, device="cuda")
+    prm = kwgpu.params(1024, 1024, layout=kwgpu.LAYOUT_WAVE_REG)
+    ctx.fd1d_price_device(d_o.data_ptr(), n, d_p.data_ptr(), d_s.data_ptr(), prm)
+    ctx.synchronize()
+    p = d_p.cpu().numpy().copy()
+    assert np.all(d_s.cpu().numpy() == 0)
+    _, c = load_golden("plain_1024_head256")
+    assert np.max(np.abs(p[:256] - c["ref_fd1d"])) <= EPS
+    ctx.fd1d_price_device(d_o.data_ptr(), n, d_p.data_ptr(), d_s.data_ptr(), prm)
+    ctx.synchronize()
+    assert np.array_equal(p, d_p.cpu().numpy())
+    e = o.copy()
+    e["e"] = 0
+    d_e = torch.from_numpy(e.view(np.uint8).copy()).cuda()
+    ctx.fd1d_price_device(d_e.data_ptr(), n, d_p.data_ptr(), d_s.data_ptr(), prm)
+    ctx.synchronize()
+    pe = d_p.cpu().numpy()
+    assert np.all(pe >= -1e-12)
+    assert np.all(p >= pe - 1e-12)
+    intrinsic = np.maximum(0, o["w"] * (o["s"] - o["k"]))
+    assert np.all(p >= intrinsic - 1e-9)

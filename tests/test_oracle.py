@@ -1,0 +1,232 @@
+"""Pins the CPU oracle (oracle/kw_oracle.cpp) before it is trusted as the checker.
+
+- against the golden vectors the compiled reference wrote (tests/golden/), to 1e-12;
+- against the known answers the reference's own tests hold (test/kwPricer_test.cpp
+  values at 1.3e-3, the QuantLib fixture test/portfolio_qdfp.csv.zst at 0.005 as in
+  test/kwPortfolio_test.cpp:52-57);
+- against mathematics: LAPACK on the tridiagonal solve, put-call parity and the
+  closed form for BS, second-order convergence of the European FD price to BS,
+  grid end points and symmetry, American >= European, V >= payoff, monotonicity;
+- and by planting plausible mistakes in a rebuilt oracle and showing that the
+  golden-vector check rejects each of them.
+"""
+from __future__ import annotations
+
+import ctypes
+import json
+import math
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import golden_meta, load_golden
+from kwgpu import workloads as W
+
+EPS_GOLDEN = 1e-12
+
+
+def test_plain_512_matches_reference_vectors():
+    o, c = load_golden("plain_512")
+    p, err = oracle.fd1d_price(o, 512, 512)
+    assert err == ""
+    assert np.max(np.abs(p - c["ref_fd1d"])) <= EPS_GOLDEN
+    p, err = oracle.fd1d_bs_price(o, 512, 512)
+    assert err == ""
+    assert np.max(np.abs(p - c["ref_fd1d_bs"])) <= EPS_GOLDEN
+    oe = o.copy()
+    oe["e"] = 0
+    assert np.max(np.abs(oracle.bs_price(oe) - c["ref_bs_euro"])) <= EPS_GOLDEN
+    p, _ = oracle.fd1d_price(oe, 512, 512)
+    assert np.max(np.abs(p - c["ref_fd1d_euro"])) <= EPS_GOLDEN
+
+
+def test_pricer_test_12_known_answers():
+    """test/kwPricer_test.cpp:37-108: all three pricers within 1.3e-3 of the stated
+    values (BS NaNs skipped), and equal to the reference's own outputs."""
+    o, c = load_golden("pricer_test_12")
+    bs = oracle.bs_price(o)
+    fd, e1 = oracle.fd1d_price(o)
+    fdbs, e2 = oracle.fd1d_bs_price(o)
+    assert e1 == "" and e2 == ""
+    ok = ~np.isnan(bs)
+    assert ok.sum() == 6 and np.all(np.isnan(c["ref_bs"][~ok]))
+    assert np.max(np.abs(bs[ok] - c["want"][ok])) <= 1.3e-3
+    assert np.max(np.abs(fd - c["want"])) <= 1.3e-3
+    assert np.max(np.abs(fdbs - c["want"])) <= 1.3e-3
+    assert np.max(np.abs(bs[ok] - c["ref_bs"][ok])) <= EPS_GOLDEN
+    assert np.max(np.abs(fd - c["ref_fd1d"])) <= EPS_GOLDEN
+    assert np.max(np.abs(fdbs - c["ref_fd1d_bs"])) <= EPS_GOLDEN
+
+
+def test_qdfp_fixture():
+    """test/kwPortfolio_test.cpp:29-58: 6000 American puts priced at 512 x 512 within
+    0.005 of QuantLib's QdFp prices; and equal to the reference FD1D output."""
+    o, c = load_golden("portfolio_qdfp")
+    assert len(o) == 6000
+    p, err = oracle.fd1d_price(o, 512, 512)
+    assert err == ""
+    assert np.max(np.abs(p - c["want"])) <= 0.005
+    assert np.max(np.abs(p - c["ref_fd1d"])) <= EPS_GOLDEN
+
+
+def test_plain_1024_head():
+    o, c = load_golden("plain_1024_head256")
+    p, err = oracle.fd1d_price(o, 1024, 1024)
+    assert err == ""
+    assert np.max(np.abs(p - c["ref_fd1d"])) <= EPS_GOLDEN
+
+
+def test_uneven_small():
+    o, c = load_golden("uneven_small")
+    p, st = oracle.fd1d_price_uneven(o, c["x_dim"], c["t_dim"])
+    assert np.all(st == 0)
+    assert np.max(np.abs(p - c["ref_fd1d"])) <= EPS_GOLDEN
+
+
+def test_range_error_message():
+    m = golden_meta()["range_error"]
+    o = W.plain(4, m["seed"])
+    o["s"] = m["spots"]
+    _, err = oracle.fd1d_price(o, m["x_dim"], m["t_dim"], m["density"], m["scale"])
+    assert err == m["message"] and err.startswith("Fd1d_Pricer::price Fd1d::value: x=")
+    _, st = oracle.fd1d_price_status(o, m["x_dim"], m["t_dim"], m["density"], m["scale"])
+    assert np.any(st == 1)
+
+
+def test_empty_portfolio():
+    p, err = oracle.fd1d_price(np.zeros(0, W.OPTION_DTYPE))
+    assert err == "" and len(p) == 0
+
+
+@pytest.mark.skipif(not oracle.ref_available(), reason="reference checker not built here")
+@pytest.mark.parametrize("seed,x,t", [(11, 100, 37), (12, 257, 64), (13, 64, 300)])
+def test_against_compiled_reference(seed, x, t):
+    o = W.plain(48, seed)
+    o["e"][::3] = 0
+    p, e1 = oracle.fd1d_price(o, x, t)
+    r, e2 = oracle.ref_price("FD1D", o, x, t)
+    assert e1 == e2
+    assert np.array_equal(p, r)
+
+
+# ---- mathematics ----------------------------------------------------------------
+
+def test_tridiagonal_matches_lapack():
+    rng = np.random.default_rng(0)
+    for n in (3, 7, 64, 513):
+        al = rng.uniform(-1, 0, n)
+        au = rng.uniform(-1, 0, n)
+        a = 2.5 + rng.uniform(0, 1, n)
+        al[0] = 0
+        au[-1] = 0
+        y = rng.normal(size=n)
+        x, rc = oracle.solve_tridiagonal(al, a, au, y)
+        assert rc == 0
+        A = np.diag(a) + np.diag(al[1:], -1) + np.diag(au[:-1], 1)
+        assert np.max(np.abs(x - np.linalg.solve(A, y))) < 1e-13
+        assert np.max(np.abs(A @ x - y)) < 1e-13
+    # error codes of src/Math/kwMath.cpp:19-24
+    assert oracle.solve_tridiagonal([0, 0, 0], [0, 1, 1], [0, 0, 0], [1, 1, 1])[1] == 1
+    assert oracle.solve_tridiagonal([0, 0], [1, 1], [0, 0], [1, 1])[1] == 3
+
+
+def _bs_closed_form(o):
+    """Textbook Black-Scholes-Merton with continuous dividend yield."""
+    out = []
+    for a in o:
+        t, k, z, r, q, s, w = (float(a[c]) for c in ("t", "k", "z", "r", "q", "s", "w"))
+        d1 = (math.log(s / k) + (r - q + 0.5 * z * z) * t) / (z * math.sqrt(t))
+        d2 = d1 - z * math.sqrt(t)
+        N = lambda v: 0.5 * math.erfc(-v / math.sqrt(2))  # noqa: E731
+        if w > 0:
+            out.append(s * math.exp(-q * t) * N(d1) - k * math.exp(-r * t) * N(d2))
+        else:
+            out.append(k * math.exp(-r * t) * N(-d2) - s * math.exp(-q * t) * N(-d1))
+    return np.array(out)
+
+
+def test_bs_closed_form_and_parity():
+    o = W.plain(64, 21)
+    o["e"] = 0
+    o["s"] = np.linspace(60, 160, 64)
+    p = oracle.bs_price(o)
+    assert np.max(np.abs(p - _bs_closed_form(o))) < 1e-10
+    c = o.copy()
+    c["w"] = 1
+    pu = o.copy()
+    pu["w"] = -1
+    lhs = oracle.bs_price(c) - oracle.bs_price(pu)
+    rhs = c["s"] * np.exp(-c["q"] * c["t"]) - c["k"] * np.exp(-c["r"] * c["t"])
+    assert np.max(np.abs(lhs - rhs)) < 1e-10
+    am = o.copy()
+    am["e"] = 1
+    assert np.all(np.isnan(oracle.bs_price(am)))
+
+
+def test_european_fd_converges_to_bs_second_order():
+    o = W.plain(16, 22)
+    o["e"] = 0
+    bs = oracle.bs_price(o)
+    errs = []
+    for n in (128, 256, 512):
+        p, err = oracle.fd1d_price(o, n, n)
+        assert err == ""
+        errs.append(np.max(np.abs(p - bs)))
+    assert errs[0] / errs[1] > 2.8 and errs[1] / errs[2] > 2.8, errs
+
+
+def test_grid_endpoints_symmetry_and_payoff_floor():
+    o = W.plain(8, 23)
+    for a in o:
+        x, v = oracle.solve_one(np.array([a]), 257, 64)
+        half = 50 * a["z"] * math.sqrt(a["t"])
+        assert abs(x[0] + half) < 1e-12 * half and abs(x[-1] - half) < 1e-12 * half
+        assert np.all(np.diff(x) > 0)
+        assert np.max(np.abs(x + x[::-1])) < 1e-12 * half
+        pay = np.maximum(0, np.exp(x) - 1) if a["w"] > 0 else np.maximum(0, 1 - np.exp(x))
+        assert np.all(v[:-1] >= pay[:-1])
+
+
+def test_american_ge_european_and_monotone_in_spot():
+    o = np.repeat(W.plain(6, 24), 9)
+    o["s"] = np.tile(np.linspace(70, 130, 9), 6)
+    am, _ = oracle.fd1d_price(o, 256, 256)
+    e = o.copy()
+    e["e"] = 0
+    eu, _ = oracle.fd1d_price(e, 256, 256)
+    assert np.all(am >= eu - 1e-12)
+    for i in range(6):
+        blk = am[9 * i: 9 * i + 9]
+        d = np.diff(blk)
+        assert np.all(d >= -1e-12) if o["w"][9 * i] > 0 else np.all(d <= 1e-12)
+
+
+# ---- planted mistakes -----------------------------------------------------------
+
+PLANTS = {
+    1: "theta 0.55 instead of the Crank-Nicolson 0.5",
+    2: "swap the lower and upper second-difference weights of the implicit rows",
+    3: "t-grid with T / tDim instead of T / (tDim - 1)",
+    4: "interpolate from the node after the bracket",
+    5: "payoff of a call for every option",
+}
+
+
+@pytest.mark.parametrize("plant", sorted(PLANTS))
+def test_golden_check_catches_planted_mistake(plant, tmp_path):
+    so = tmp_path / f"liboracle_plant{plant}.so"
+    subprocess.check_call(["g++", "-std=c++20", "-O2", "-fPIC", "-shared", f"-DKWO_PLANT={plant}",
+                           "-o", str(so), os.path.join(oracle.HERE, "kw_oracle.cpp"),
+                           "-lpthread"])
+    L = ctypes.CDLL(str(so))
+    P, SZ, D, I64 = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_double, ctypes.c_int64
+    L.kwo_fd1d_price.argtypes = [P, SZ, D, D, I64, I64, ctypes.c_int, P, ctypes.c_char_p, SZ]
+    o, c = load_golden("plain_512")
+    out = np.zeros(len(o))
+    err = ctypes.create_string_buffer(256)
+    L.kwo_fd1d_price(np.ascontiguousarray(o).ctypes.data, len(o), 0.25, 50.0, 512, 512, 0,
+                     out.ctypes.data, err, 256)
+    assert np.max(np.abs(out - c["ref_fd1d"])) > 1e-8, PLANTS[plant]

@@ -99,34 +99,50 @@ def main():
     d_status = torch.empty(N_OPTIONS, dtype=torch.int32, device=dev)
     ctx = kwgpu.Context(0)
     stream = torch.cuda.current_stream(dev)
-    prm = kwgpu.params(X_DIM, T_DIM, precision=kwgpu.FP64, layout=layout)
 
-    def step():
-        ctx.fd1d_price_device(d_opts.data_ptr(), N_OPTIONS, d_prices.data_ptr(),
-                              d_status.data_ptr(), prm, stream=stream.cuda_stream)
+    def run_form(precision, steps, warmup):
+        """Times `steps` launches of one form; returns (ms/step wall, kernel ms list, prices)."""
+        prm = kwgpu.params(X_DIM, T_DIM, precision=precision, layout=layout)
 
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    print(f"bench: warmup done ({args.warmup} steps)", file=sys.stderr, flush=True)
+        def step():
+            ctx.fd1d_price_device(d_opts.data_ptr(), N_OPTIONS, d_prices.data_ptr(),
+                                  d_status.data_ptr(), prm, stream=stream.cuda_stream)
 
-    # HIP events around every launch on the launch stream
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    ev[0].record(stream)
-    for i in range(args.steps):
-        step()
-        ev[i + 1].record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    kernel_ms = [ev[i].elapsed_time(ev[i + 1]) for i in range(args.steps)]
-    ms_per_step = wall * 1e3 / args.steps
-    status = d_status.cpu().numpy()
-    if not np.all(status == 0):
-        raise RuntimeError(f"{int(np.sum(status != 0))} options failed")
-    print(f"bench: timed {args.steps} steps, {ms_per_step:.3f} ms/step", file=sys.stderr,
+        for _ in range(warmup):
+            step()
+        torch.cuda.synchronize()
+        # HIP events around every launch on the launch stream
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ev[0].record(stream)
+        for i in range(steps):
+            step()
+            ev[i + 1].record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        kms = [ev[i].elapsed_time(ev[i + 1]) for i in range(steps)]
+        status = d_status.cpu().numpy()
+        if not np.all(status == 0):
+            raise RuntimeError(f"{int(np.sum(status != 0))} options failed")
+        return wall * 1e3 / steps, kms, d_prices.cpu().numpy().copy()
+
+    ms_per_step, kernel_ms, p64 = run_form(kwgpu.FP64, args.steps, args.warmup)
+    print(f"bench: FP64 timed {args.steps} steps, {ms_per_step:.3f} ms/step", file=sys.stderr,
           flush=True)
+    forms = {"fd_fp64": {"options_per_s": N_OPTIONS / (ms_per_step * 1e-3),
+                         "ms_per_step": ms_per_step, "max_abs_err_vs_fp64_fd": 0.0}}
+    if args.layout != "lane":
+        ms_mx, kms_mx, pmx = run_form(kwgpu.FP32_MIXED, args.steps, 1)
+        d = np.abs(pmx - p64)
+        forms["fd_fp32_mixed"] = {
+            "options_per_s": N_OPTIONS / (ms_mx * 1e-3), "ms_per_step": ms_mx,
+            "kernel_ms": float(np.mean(kms_mx)),
+            "max_abs_err_vs_fp64_fd": float(d.max()),
+            "rms_abs_err_vs_fp64_fd": float(np.sqrt(np.mean(d * d))),
+            "speedup_vs_fp64": ms_per_step / ms_mx}
+        print(f"bench: FP32-mixed {ms_mx:.3f} ms/step, max err {d.max():.2e}", file=sys.stderr,
+              flush=True)
 
     if args.copy_probe_mib:
         nbytes = args.copy_probe_mib << 20
@@ -147,6 +163,8 @@ def main():
         print(f"bench: CPU baseline, {args.cpu_sample} options on {threads} threads",
               file=sys.stderr, flush=True)
         cpu = cpu_baseline(opts, args.cpu_sample, threads)
+        for f in forms.values():
+            f["speedup_vs_cpu"] = f["options_per_s"] / cpu["value"]
 
     out = {
         "metric": "options/sec at max abs err <=1e-4 vs FP64 FD: plain FP64 vs FP32 vs surrogate",
@@ -167,6 +185,7 @@ def main():
             "form": "fd_fp64",
             "layout": args.layout,
             "max_abs_err_vs_fp64_fd": 0.0,
+            "forms": forms,
         },
         "roofline": {
             "bound": "mfma",

@@ -4,8 +4,9 @@
 // (src/Math/kwFd1d.cpp:11-158, src/Math/kwMath.cpp:16-49) for a batch of PDEs.
 //
 // Layout: one PDE per L-lane segment of a wave64 (L = 16, 32 or 64, so 4, 2 or 1
-// options per wave); lane l of a segment owns grid rows [l*M, l*M + M). Padding rows
-// beyond x_dim are identity rows and stay zero.
+// options per wave), or per workgroup of W = 2 whole waves for 2048-point grids; lane
+// l of a segment owns grid rows [l*M, l*M + M). Padding rows beyond x_dim are identity
+// rows and stay zero.
 //
 // The operator is time-invariant (constant coefficients, uniform t-grid), so B = I -
 // theta dt A is LU-factored once in FP64. Since E = I + (1-theta) dt A =
@@ -22,34 +23,26 @@
 // row_shl and readlane backward) whose multipliers depend only on B and so are
 // precomputed; per-row fix-up coefficients fold the carries back in.
 //
-// Per point and step that is 7 instructions (2 forward, 1 backward, 3 fix-up and
+// FP64 march: 7 instructions per point and step (2 forward, 1 backward, 3 fix-up and
 // correction, 1 projection) plus the scans amortised over M rows. The state, the
 // factors and the forward fix-up live in VGPRs; the backward fix-up and the payoff
 // live in LDS (one ds_read_b128 per row and step) so that a wave stays within 256
 // registers and two waves share each SIMD.
+//
+// FP32-mixed march: the same factorisation applied to the step's increment,
+//     V' - V = (1/theta) B^{-1} (I - B) V,
+// with the residual (I - B) V assembled in FP64 from FP64 differences of V (so its
+// rounding error is relative to the increment, not to V), the substitutions, carries
+// and fix-ups in FP32 on FP32 copies of the factors, and V accumulated, projected and
+// interpolated in FP64.
 #include <math.h>
+
+#include <algorithm>
 
 #include "kwgpu_internal.h"
 
 namespace kwgpu {
 namespace {
-
-// 64-bit DPP move built from two 32-bit v_mov_b32_dpp. Lanes whose source is outside
-// the row/wave read 0 (bound_ctrl); lanes disabled by ROWMASK read 0.
-template <int CTRL, int ROWMASK = 0xF>
-__device__ __forceinline__ double dpp(double v) {
-    int2 a = __builtin_bit_cast(int2, v);
-    a.x = __builtin_amdgcn_update_dpp(0, a.x, CTRL, ROWMASK, 0xF, true);
-    a.y = __builtin_amdgcn_update_dpp(0, a.y, CTRL, ROWMASK, 0xF, true);
-    return __builtin_bit_cast(double, a);
-}
-template <int LANE>
-__device__ __forceinline__ double readlane(double v) {
-    int2 a = __builtin_bit_cast(int2, v);
-    a.x = __builtin_amdgcn_readlane(a.x, LANE);
-    a.y = __builtin_amdgcn_readlane(a.y, LANE);
-    return __builtin_bit_cast(double, a);
-}
 
 constexpr int kRowShr = 0x110;   // + n, n in 1..15
 constexpr int kRowShl = 0x100;   // + n
@@ -57,6 +50,32 @@ constexpr int kRowBcast15 = 0x142;
 constexpr int kRowBcast31 = 0x143;
 constexpr int kWaveShr1 = 0x138;
 constexpr int kWaveShl1 = 0x130;
+
+// DPP move of a float or a double (two 32-bit v_mov_b32_dpp). Lanes whose source is
+// outside the row/wave read 0 (bound_ctrl); lanes disabled by ROWMASK read 0.
+template <int CTRL, int ROWMASK = 0xF, class T>
+__device__ __forceinline__ T dpp(T v) {
+    if constexpr (sizeof(T) == 8) {
+        int2 a = __builtin_bit_cast(int2, v);
+        a.x = __builtin_amdgcn_update_dpp(0, a.x, CTRL, ROWMASK, 0xF, true);
+        a.y = __builtin_amdgcn_update_dpp(0, a.y, CTRL, ROWMASK, 0xF, true);
+        return __builtin_bit_cast(T, a);
+    } else {
+        return __builtin_bit_cast(
+            T, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, ROWMASK, 0xF, true));
+    }
+}
+template <int LANE, class T>
+__device__ __forceinline__ T readlane(T v) {
+    if constexpr (sizeof(T) == 8) {
+        int2 a = __builtin_bit_cast(int2, v);
+        a.x = __builtin_amdgcn_readlane(a.x, LANE);
+        a.y = __builtin_amdgcn_readlane(a.y, LANE);
+        return __builtin_bit_cast(T, a);
+    } else {
+        return __builtin_bit_cast(T, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), LANE));
+    }
+}
 
 // max(a, b) as one v_max_f64: fmax() would also re-canonicalise its operands every
 // step (LLVM maxnum semantics), doubling the projection's cost.
@@ -73,42 +92,88 @@ __device__ __forceinline__ double lds_prod(const double* v, int from, int to) {
     return p;
 }
 
-template <int M, int L>
-__global__ __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves_per_eu(2, 2)))
-void fd1d_wave_f64(FdBatch b) {
+template <int M, int L, int W>
+struct Geo {
     static_assert(L == 16 || L == 32 || L == 64, "segment width");
+    static_assert(W == 1 || (W == 2 && L == 64), "two-wave groups use whole waves");
     static_assert(M >= 4, "rows per lane");
-    constexpr int G = 64 / L;
-    // One LDS region per wave, used three ways in turn:
-    //   set-up scratch (256 doubles), then during the march the pairs
-    //   (Q_i, payoff_i) of every lane at qp[(i * 64 + lane) * 2], then the solution
-    //   for the interpolation at qp[seg * L * M + j].
-    __shared__ double qp[2 * 64 * M];
+    static constexpr int NT = 64 * W;           // threads per workgroup
+    static constexpr int LT = W == 1 ? L : NT;  // lanes per option
+    static constexpr int G = NT / LT;           // options per workgroup
+};
 
-    const int lane = threadIdx.x;
-    const int seg = lane / L;
-    const int ls = lane - seg * L;
-    const int row16 = lane >> 4;
-    const int pr = lane & 15;
-    const int slot = blockIdx.x * G + seg;
-    const bool live = slot < b.m;
-    const int p = live ? (b.order ? (int)b.order[slot] : slot) : 0;
-    const uint32_t oi = live ? (b.rep ? b.rep[p] : (uint32_t)p) : 0u;
-    const kwgpu_option opt = b.opts[oi];
-    const int xd = b.x_dims ? b.x_dims[oi] : b.x_dim;
-    const int td = b.t_dims ? b.t_dims[oi] : b.t_dim;
-    const PdeParams P = pde_params(opt, xd, td, b.density, b.scale);
-    const bool solve = live && P.valid && xd <= L * M;
-    const int N = solve ? xd : 0;  // N == 0: every row is an identity row
-    const int steps = solve ? td - 1 : 0;
+struct Lanes {
+    int gl, wv, lane, seg, ls, row16, wrow, pr;
+};
+
+template <class T>
+struct ScanMul {
+    T F1, F2, F4, F8, FB15, FB31;  // forward, window products of the lanes' P
+    T G1, G2, G4, G8, XB;          // backward, window products of the lanes' Q
+    T FW, ZW;                      // two-wave groups: carries between the waves
+};
+
+// Per-PDE state after set-up.
+struct Pde {
+    PdeParams P;
+    int p;          // PDE index
+    uint32_t oi;    // representative option
+    kwgpu_option opt;
+    bool live, solve;
+    int N, steps, j0;
+};
+
+template <int M, int L, int W>
+__device__ __forceinline__ Lanes lanes_of(int gl) {
+    using Gm = Geo<M, L, W>;
+    Lanes t;
+    t.gl = gl;
+    t.wv = gl >> 6;
+    t.lane = gl & 63;
+    t.seg = gl / Gm::LT;
+    t.ls = gl - t.seg * Gm::LT;
+    t.row16 = gl >> 4;
+    t.wrow = t.lane >> 4;
+    t.pr = gl & 15;
+    return t;
+}
+
+template <int M, int L, int W>
+__device__ __forceinline__ Pde pde_of(const FdBatch& b, int group, const Lanes& t) {
+    using Gm = Geo<M, L, W>;
+    Pde d;
+    const int slot = group * Gm::G + t.seg;
+    d.live = slot < b.m;
+    d.p = d.live ? (b.order ? (int)b.order[slot] : slot) : 0;
+    d.oi = d.live ? (b.rep ? b.rep[d.p] : (uint32_t)d.p) : 0u;
+    d.opt = b.opts[d.oi];
+    const int xd = b.x_dims ? b.x_dims[d.oi] : b.x_dim;
+    const int td = b.t_dims ? b.t_dims[d.oi] : b.t_dim;
+    d.P = pde_params(d.opt, xd, td, b.density, b.scale);
+    d.solve = d.live && d.P.valid && xd <= Gm::LT * M;
+    d.N = d.solve ? xd : 0;  // N == 0: every row is an identity row
+    d.steps = d.solve ? td - 1 : 0;
+    d.j0 = t.ls * M;
+    return d;
+}
+
+// ---- set-up: grid, operator rows, payoff, LU factors, carry coefficients (FP64) ----
+// qp: LDS scratch of at least 4 * NT doubles. On return xv holds the payoff, ap/bt/gm
+// the scaled LU factors, R/Q the carry fix-ups, blv/buv the off-diagonals of B, and
+// sm the scan multipliers.
+template <int M, int L, int W>
+__device__ __forceinline__ void setup(const FdBatch& b, const Pde& d, const Lanes& t, double* qp,
+                                      double (&xv)[M], double (&ap)[M], double (&bt)[M],
+                                      double (&gm)[M], double (&R)[M], double (&Q)[M],
+                                      double (&blv)[M], double (&buv)[M], ScanMul<double>& sm) {
+    using Gm = Geo<M, L, W>;
+    constexpr int NT = Gm::NT, LT = Gm::LT;
+    const PdeParams& P = d.P;
+    const int N = d.N, j0 = d.j0, gl = t.gl;
     const double th = b.theta;
     const double inv_th = 1.0 / th;
-    const double c_th = (1.0 - th) / th;
-    const int j0 = ls * M;
-
-    // ---- set-up: grid, operator rows, payoff, LU factors (FP64) --------------------
-    double xv[M], ap[M], bt[M], gm[M], R[M];
-    double Q[M];  // only during set-up; the march reads it from LDS
+    double bv[M];
+    double bu_prev = 0.0;  // bu of row j0-1
     {
         double xg[M + 3];  // x_{j0-2} .. x_{j0+M}
 #pragma unroll
@@ -116,8 +181,6 @@ void fd1d_wave_f64(FdBatch b) {
             const int j = j0 - 2 + i;
             xg[i] = (j >= 0 && j < N) ? grid_x(P, b.density, j) : 0.0;
         }
-        double blv[M], bv[M], buv[M];
-        double bu_prev = 0.0;  // bu of row j0-1
         if (j0 >= 1 && j0 - 1 < N) {
             double t0, t1;
             b_row(P, th, j0 - 1, xg[0], xg[1], xg[2], t0, t1, bu_prev);
@@ -128,54 +191,55 @@ void fd1d_wave_f64(FdBatch b) {
             b_row(P, th, j, xg[i + 1], xg[i + 2], xg[i + 3], blv[i], bv[i], buv[i]);
             xv[i] = j < N ? payoff(P, xg[i + 2]) : 0.0;
         }
+    }
 
-        // LU pivots d_j = b_j - bl_j bu_{j-1} / d_{j-1} are a serial recurrence; its
-        // value entering each lane comes from a prefix product of the lanes' projective
-        // maps (u, v) -> (b u - bl bu_prev v, u), normalised to avoid overflow.
-        double m00 = 1, m01 = 0, m10 = 0, m11 = 1;
-        {
-            double bup = bu_prev;
+    // LU pivots d_j = b_j - bl_j bu_{j-1} / d_{j-1} are a serial recurrence; its value
+    // entering each lane comes from a prefix product of the lanes' projective maps
+    // (u, v) -> (b u - bl bu_prev v, u), normalised to avoid overflow.
+    double m00 = 1, m01 = 0, m10 = 0, m11 = 1;
+    {
+        double bup = bu_prev;
 #pragma unroll
-            for (int i = 0; i < M; ++i) {
-                const double e = blv[i] * bup;
-                const double n00 = bv[i] * m00 - e * m10, n01 = bv[i] * m01 - e * m11;
-                m10 = m00;
-                m11 = m01;
-                m00 = n00;
-                m01 = n01;
-                const double s = fmax(fmax(fabs(m00), fabs(m01)), fmax(fabs(m10), fabs(m11)));
-                const double is = s > 0.0 ? 1.0 / s : 1.0;
-                m00 *= is; m01 *= is; m10 *= is; m11 *= is;
-                bup = buv[i];
-            }
-        }
-        qp[4 * lane + 0] = m00;
-        qp[4 * lane + 1] = m01;
-        qp[4 * lane + 2] = m10;
-        qp[4 * lane + 3] = m11;
-        __syncthreads();
-        double s00 = 1, s01 = 0, s10 = 0, s11 = 1;
-        for (int k = seg * L; k < lane; ++k) {
-            const double a00 = qp[4 * k], a01 = qp[4 * k + 1], a10 = qp[4 * k + 2],
-                         a11 = qp[4 * k + 3];
-            const double n00 = a00 * s00 + a01 * s10, n01 = a00 * s01 + a01 * s11;
-            const double n10 = a10 * s00 + a11 * s10, n11 = a10 * s01 + a11 * s11;
-            const double s = fmax(fmax(fabs(n00), fabs(n01)), fmax(fabs(n10), fabs(n11)));
+        for (int i = 0; i < M; ++i) {
+            const double e = blv[i] * bup;
+            const double n00 = bv[i] * m00 - e * m10, n01 = bv[i] * m01 - e * m11;
+            m10 = m00;
+            m11 = m01;
+            m00 = n00;
+            m01 = n01;
+            const double s = fmax(fmax(fabs(m00), fabs(m01)), fmax(fabs(m10), fabs(m11)));
             const double is = s > 0.0 ? 1.0 / s : 1.0;
-            s00 = n00 * is; s01 = n01 * is; s10 = n10 * is; s11 = n11 * is;
+            m00 *= is; m01 *= is; m10 *= is; m11 *= is;
+            bup = buv[i];
         }
-        double dprev = ls == 0 ? 1.0 : (s00 + s01) / (s10 + s11);
-
+    }
+    __syncthreads();  // the previous group's interpolation may still read qp
+    qp[4 * gl + 0] = m00;
+    qp[4 * gl + 1] = m01;
+    qp[4 * gl + 2] = m10;
+    qp[4 * gl + 3] = m11;
+    __syncthreads();
+    double s00 = 1, s01 = 0, s10 = 0, s11 = 1;
+    for (int k = t.seg * LT; k < gl; ++k) {
+        const double a00 = qp[4 * k], a01 = qp[4 * k + 1], a10 = qp[4 * k + 2], a11 = qp[4 * k + 3];
+        const double n00 = a00 * s00 + a01 * s10, n01 = a00 * s01 + a01 * s11;
+        const double n10 = a10 * s00 + a11 * s10, n11 = a10 * s01 + a11 * s11;
+        const double s = fmax(fmax(fabs(n00), fabs(n01)), fmax(fabs(n10), fabs(n11)));
+        const double is = s > 0.0 ? 1.0 / s : 1.0;
+        s00 = n00 * is; s01 = n01 * is; s10 = n10 * is; s11 = n11 * is;
+    }
+    {
+        double dprev = t.ls == 0 ? 1.0 : (s00 + s01) / (s10 + s11);
         // Thomas pivots of the reference (gam_j = bu_{j-1}/bet, bet = b_j - bl_j gam_j)
         double bup = bu_prev;
 #pragma unroll
         for (int i = 0; i < M; ++i) {
             const double gam = bup / dprev;
-            const double d = bv[i] - blv[i] * gam;
-            ap[i] = inv_th / d;
-            bt[i] = blv[i] / d;
-            gm[i] = buv[i] / d;
-            dprev = d;
+            const double dd = bv[i] - blv[i] * gam;
+            ap[i] = inv_th / dd;
+            bt[i] = blv[i] / dd;
+            gm[i] = buv[i] / dd;
+            dprev = dd;
             bup = buv[i];
         }
     }
@@ -198,154 +262,379 @@ void fd1d_wave_f64(FdBatch b) {
         Ql = Q[0];
     }
     __syncthreads();
-    qp[lane] = Pl;
-    qp[64 + lane] = Ql;
+    qp[gl] = Pl;
+    qp[NT + gl] = Ql;
     __syncthreads();
-    // Forward scan multipliers (window products of Pl over lanes).
-    const double F1 = pr >= 1 ? lds_prod(qp, lane, lane) : 0.0;
-    const double F2 = pr >= 2 ? lds_prod(qp, lane - 1, lane) : 0.0;
-    const double F4 = pr >= 4 ? lds_prod(qp, lane - 3, lane) : 0.0;
-    const double F8 = pr >= 8 ? lds_prod(qp, lane - 7, lane) : 0.0;
-    const double FB15 = (L >= 32 && (row16 & 1)) ? lds_prod(qp, 16 * row16, lane) : 0.0;
-    const double FB31 = (L == 64 && row16 >= 2) ? lds_prod(qp, 32, lane) : 0.0;
-    // Backward scan multipliers (window products of Ql).
-    const double* lq = qp + 64;
-    const double G1 = pr + 1 <= 15 ? lds_prod(lq, lane, lane) : 0.0;
-    const double G2 = pr + 2 <= 15 ? lds_prod(lq, lane, lane + 1) : 0.0;
-    const double G4 = pr + 4 <= 15 ? lds_prod(lq, lane, lane + 3) : 0.0;
-    const double G8 = pr + 8 <= 15 ? lds_prod(lq, lane, lane + 7) : 0.0;
-    const double XB = lds_prod(lq, lane, 16 * row16 + 15);
-    // Segment edges: selects, not multiplies by 0, so one segment's values never
-    // reach another's.
-    const bool first_lane = ls == 0;
-    const bool last_lane = ls == L - 1;
+    const int pr = t.pr, row16 = t.row16, wrow = t.wrow, wv = t.wv;
+    sm.F1 = pr >= 1 ? lds_prod(qp, gl, gl) : 0.0;
+    sm.F2 = pr >= 2 ? lds_prod(qp, gl - 1, gl) : 0.0;
+    sm.F4 = pr >= 4 ? lds_prod(qp, gl - 3, gl) : 0.0;
+    sm.F8 = pr >= 8 ? lds_prod(qp, gl - 7, gl) : 0.0;
+    sm.FB15 = (L >= 32 && (wrow & 1)) ? lds_prod(qp, 16 * row16, gl) : 0.0;
+    sm.FB31 = (L == 64 && wrow >= 2) ? lds_prod(qp, 64 * wv + 32, gl) : 0.0;
+    const double* lq = qp + NT;
+    sm.G1 = pr + 1 <= 15 ? lds_prod(lq, gl, gl) : 0.0;
+    sm.G2 = pr + 2 <= 15 ? lds_prod(lq, gl, gl + 1) : 0.0;
+    sm.G4 = pr + 4 <= 15 ? lds_prod(lq, gl, gl + 3) : 0.0;
+    sm.G8 = pr + 8 <= 15 ? lds_prod(lq, gl, gl + 7) : 0.0;
+    sm.XB = lds_prod(lq, gl, 16 * row16 + 15);
+    // two-wave groups: carry of wave 0's last lane into wave 1, and of wave 1's first
+    // lane back into wave 0
+    sm.FW = (W == 2 && wv == 1) ? lds_prod(qp, 64, gl) : 0.0;
+    sm.ZW = (W == 2 && wv == 0) ? lds_prod(lq, gl, 63) : 0.0;
+}
 
-    // All segments of the wave iterate to the wave's longest march; lanes whose own
-    // march is done keep their values.
-    int wsteps = steps;
-    if (L < 64) {
+// All segments of a workgroup iterate to its longest march; lanes whose own march is
+// done keep their values.
+template <int M, int L, int W>
+__device__ __forceinline__ int group_steps(const Pde& d, const Lanes& t, double* qp) {
+    using Gm = Geo<M, L, W>;
+    if constexpr (Gm::G == 1) {
+        return d.steps;
+    } else {
         __syncthreads();
-        qp[lane] = (double)steps;
+        qp[t.gl] = (double)d.steps;
         __syncthreads();
         double mx = 0.0;
-        for (int k = 0; k < 64; k += L) mx = fmax(mx, qp[k]);
-        wsteps = (int)mx;
+        for (int k = 0; k < Gm::NT; k += Gm::LT) mx = fmax(mx, qp[k]);
+        return (int)mx;
     }
+}
+
+// Carries of one step: forward scan of `c` (the lanes' local last values) giving each
+// lane's f_in; `xchg` links the two waves of a W = 2 group.
+template <int L, int W, class T>
+__device__ __forceinline__ T fwd_carry(T c, const ScanMul<T>& sm, const Lanes& t, bool first_lane,
+                                       T* xchg) {
+    c = fma(sm.F1, dpp<kRowShr + 1>(c), c);
+    c = fma(sm.F2, dpp<kRowShr + 2>(c), c);
+    c = fma(sm.F4, dpp<kRowShr + 4>(c), c);
+    c = fma(sm.F8, dpp<kRowShr + 8>(c), c);
+    if constexpr (L >= 32) c = fma(sm.FB15, dpp<kRowBcast15, 0xA>(c), c);
+    if constexpr (L == 64) c = fma(sm.FB31, dpp<kRowBcast31, 0xC>(c), c);
+    T fin;
+    if constexpr (W == 2) {
+        if (t.wv == 0 && t.lane == 63) xchg[0] = c;
+        __syncthreads();
+        if (t.wv == 1) {
+            const T c0 = xchg[0];
+            c = fma(sm.FW, c0, c);
+            fin = dpp<kWaveShr1>(c);
+            fin = t.lane == 0 ? c0 : fin;
+        } else {
+            fin = dpp<kWaveShr1>(c);
+        }
+    } else {
+        fin = dpp<kWaveShr1>(c);
+        // a select, not a multiply by 0, so one segment's values never reach another's
+        if constexpr (L < 64) fin = first_lane ? T(0) : fin;
+    }
+    return fin;
+}
+
+// Backward scan of `cb` (the lanes' local first values with the forward carry folded
+// in) giving each lane's z_in.
+template <int L, int W, class T>
+__device__ __forceinline__ T bwd_carry(T cb, const ScanMul<T>& sm, const Lanes& t, bool last_lane,
+                                       T* xchg) {
+    cb = fma(sm.G1, dpp<kRowShl + 1>(cb), cb);
+    cb = fma(sm.G2, dpp<kRowShl + 2>(cb), cb);
+    cb = fma(sm.G4, dpp<kRowShl + 4>(cb), cb);
+    cb = fma(sm.G8, dpp<kRowShl + 8>(cb), cb);
+    const int wrow = t.wrow;
+    if constexpr (L == 64) {
+        T u = fma(sm.XB, readlane<48>(cb), cb);
+        cb = wrow == 2 ? u : cb;
+        u = fma(sm.XB, readlane<32>(cb), cb);
+        cb = wrow == 1 ? u : cb;
+        u = fma(sm.XB, readlane<16>(cb), cb);
+        cb = wrow == 0 ? u : cb;
+    } else if constexpr (L == 32) {
+        const T u0 = fma(sm.XB, readlane<16>(cb), cb);
+        const T u2 = fma(sm.XB, readlane<48>(cb), cb);
+        cb = wrow == 0 ? u0 : (wrow == 2 ? u2 : cb);
+    }
+    T zin;
+    if constexpr (W == 2) {
+        if (t.wv == 1 && t.lane == 0) xchg[1] = cb;
+        __syncthreads();
+        if (t.wv == 0) {
+            const T z1 = xchg[1];
+            cb = fma(sm.ZW, z1, cb);
+            zin = dpp<kWaveShl1>(cb);
+            zin = t.lane == 63 ? z1 : zin;
+        } else {
+            zin = dpp<kWaveShl1>(cb);
+        }
+    } else {
+        zin = dpp<kWaveShl1>(cb);
+        if constexpr (L < 64) zin = last_lane ? T(0) : zin;
+    }
+    return zin;
+}
+
+// ---- Fd1d::value: interpolation at log(S/K), price = K v ---------------------------
+template <int M, int L, int W>
+__device__ __forceinline__ void interpolate(const FdBatch& b, const Pde& d, const Lanes& t,
+                                            const double (&xv)[M], double* qp) {
+    using Gm = Geo<M, L, W>;
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < M; ++i) qp[t.seg * Gm::LT * M + d.j0 + i] = xv[i];
+    __syncthreads();
+    if (!d.live) return;
+    const double* V = qp + t.seg * Gm::LT * M;
+    const uint32_t beg = b.off ? b.off[d.p] : 0u;
+    const uint32_t end = b.off ? b.off[d.p + 1] : 1u;
+    for (uint32_t k = beg + t.ls; k < end; k += Gm::LT) {
+        const uint32_t o2 = b.off ? b.list[k] : d.oi;
+        const kwgpu_option q = b.off ? b.opts[o2] : d.opt;
+        double price = NAN;
+        int st = KWGPU_STATUS_BAD_INPUT;
+        if (d.solve && finite_pos(q.s) && finite_pos(q.k)) {
+            const double xs = log(q.s / q.k);
+            const int j = bracket(d.P, b.density, xs);
+            if (j == 0 || j == d.N) {
+                st = KWGPU_STATUS_OUT_OF_GRID;
+            } else {
+                const double x1 = grid_x(d.P, b.density, j), x0 = grid_x(d.P, b.density, j - 1);
+                const double v = ((x1 - xs) * V[j - 1] + (xs - x0) * V[j]) / (x1 - x0);
+                price = q.k * v;
+                st = isfinite(price) ? KWGPU_STATUS_OK : KWGPU_STATUS_NONFINITE;
+            }
+        }
+        b.prices[o2] = price;
+        if (b.status) b.status[o2] = st;
+    }
+}
+
+// ---- FP64 march (src/Math/kwFd1d.cpp:67-133) -----------------------------------------
+template <int M, int L, int W>
+__device__ __forceinline__ void solve_group_f64(const FdBatch& b, int group, double* qp,
+                                                double* xchg) {
+    using Gm = Geo<M, L, W>;
+    constexpr int NT = Gm::NT;
+    const Lanes t = lanes_of<M, L, W>(threadIdx.x);
+    const Pde d = pde_of<M, L, W>(b, group, t);
+    const double c_th = (1.0 - b.theta) / b.theta;
+    double xv[M], ap[M], bt[M], gm[M], R[M], Q[M], blv[M], buv[M];
+    ScanMul<double> sm;
+    setup<M, L, W>(b, d, t, qp, xv, ap, bt, gm, R, Q, blv, buv, sm);
+    const bool first_lane = t.ls == 0;
+    const bool last_lane = t.ls == Gm::LT - 1;
+    const int wsteps = group_steps<M, L, W>(d, t, qp);
+
+    // (Q_i, projection target_i) pairs of every thread at qpv[i * NT + gl]
     __syncthreads();
     double2* qpv = reinterpret_cast<double2*>(qp);
 #pragma unroll
     for (int i = 0; i < M; ++i) {
-        const int j = j0 + i;
-        qpv[i * 64 + lane] = make_double2(Q[i], (P.american && j < N - 1) ? xv[i] : -INFINITY);
+        const bool proj = d.P.american && d.j0 + i < d.N - 1;
+        qpv[i * NT + t.gl] = make_double2(Q[i], proj ? xv[i] : -INFINITY);
     }
     __syncthreads();
 
-    // ---- march: tDim-1 backward steps (src/Math/kwFd1d.cpp:67-133) ----------------
     for (int step = 0; step < wsteps; ++step) {
         double f[M];
         f[0] = ap[0] * xv[0];
 #pragma unroll
         for (int i = 1; i < M; ++i) f[i] = fma(-bt[i], f[i - 1], ap[i] * xv[i]);
-
-        double c = f[M - 1];
-        c = fma(F1, dpp<kRowShr + 1>(c), c);
-        c = fma(F2, dpp<kRowShr + 2>(c), c);
-        c = fma(F4, dpp<kRowShr + 4>(c), c);
-        c = fma(F8, dpp<kRowShr + 8>(c), c);
-        if constexpr (L >= 32) c = fma(FB15, dpp<kRowBcast15, 0xA>(c), c);
-        if constexpr (L == 64) c = fma(FB31, dpp<kRowBcast31, 0xC>(c), c);
-        double fin = dpp<kWaveShr1>(c);
-        if constexpr (L < 64) fin = first_lane ? 0.0 : fin;
-
+        const double c = f[M - 1];
 #pragma unroll
         for (int i = M - 2; i >= 0; --i) f[i] = fma(-gm[i], f[i + 1], f[i]);
-
-        double cb = fma(R[0], fin, f[0]);
-        cb = fma(G1, dpp<kRowShl + 1>(cb), cb);
-        cb = fma(G2, dpp<kRowShl + 2>(cb), cb);
-        cb = fma(G4, dpp<kRowShl + 4>(cb), cb);
-        cb = fma(G8, dpp<kRowShl + 8>(cb), cb);
-        if constexpr (L == 64) {
-            double t = fma(XB, readlane<48>(cb), cb);
-            cb = row16 == 2 ? t : cb;
-            t = fma(XB, readlane<32>(cb), cb);
-            cb = row16 == 1 ? t : cb;
-            t = fma(XB, readlane<16>(cb), cb);
-            cb = row16 == 0 ? t : cb;
-        } else if constexpr (L == 32) {
-            const double t0 = fma(XB, readlane<16>(cb), cb);
-            const double t2 = fma(XB, readlane<48>(cb), cb);
-            cb = row16 == 0 ? t0 : (row16 == 2 ? t2 : cb);
-        }
-        double zin = dpp<kWaveShl1>(cb);
-        if constexpr (L < 64) zin = last_lane ? 0.0 : zin;
-
-        if (step < steps) {
+        const double fin = fwd_carry<L, W>(c, sm, t, first_lane, xchg);
+        const double zin = bwd_carry<L, W>(fma(R[0], fin, f[0]), sm, t, last_lane, xchg);
+        if (step < d.steps) {
 #pragma unroll
             for (int i = 0; i < M; ++i) {
-                const double2 qv = qpv[i * 64 + lane];
-                double t = fma(R[i], fin, f[i]);
-                t = fma(qv.x, zin, t);
-                xv[i] = vmax(fma(-c_th, xv[i], t), qv.y);
+                const double2 qv = qpv[i * NT + t.gl];
+                double u = fma(R[i], fin, f[i]);
+                u = fma(qv.x, zin, u);
+                xv[i] = vmax(fma(-c_th, xv[i], u), qv.y);
             }
         }
     }
+    interpolate<M, L, W>(b, d, t, xv, qp);
+}
 
-    // ---- Fd1d::value: interpolation at log(S/K), price = K v -----------------------
+// ---- FP32-mixed march ---------------------------------------------------------------
+template <int M, int L, int W>
+__device__ __forceinline__ void solve_group_mixed(const FdBatch& b, int group, double* qp,
+                                                  double* xchg) {
+    using Gm = Geo<M, L, W>;
+    constexpr int NT = Gm::NT;
+    const Lanes t = lanes_of<M, L, W>(threadIdx.x);
+    const Pde d = pde_of<M, L, W>(b, group, t);
+    double xv[M], blv[M], buv[M];
+    float ap[M], bt[M], gm[M], R[M], Q[M];
+    ScanMul<float> sm;
+    {
+        double ap64[M], bt64[M], gm64[M], R64[M], Q64[M];
+        ScanMul<double> s64;
+        setup<M, L, W>(b, d, t, qp, xv, ap64, bt64, gm64, R64, Q64, blv, buv, s64);
+#pragma unroll
+        for (int i = 0; i < M; ++i) {
+            ap[i] = (float)ap64[i];
+            bt[i] = (float)bt64[i];
+            gm[i] = (float)gm64[i];
+            R[i] = (float)R64[i];
+            Q[i] = (float)Q64[i];
+        }
+        sm.F1 = (float)s64.F1; sm.F2 = (float)s64.F2; sm.F4 = (float)s64.F4;
+        sm.F8 = (float)s64.F8; sm.FB15 = (float)s64.FB15; sm.FB31 = (float)s64.FB31;
+        sm.G1 = (float)s64.G1; sm.G2 = (float)s64.G2; sm.G4 = (float)s64.G4;
+        sm.G8 = (float)s64.G8; sm.XB = (float)s64.XB; sm.FW = (float)s64.FW;
+        sm.ZW = (float)s64.ZW;
+    }
+    // diagonal of (I - B) is theta dt a0 on every real row (the stencils' weights sum
+    // to zero); padding rows hold V = 0
+    const double sdiag = b.theta * d.P.dt * d.P.a0;
+    const bool first_lane = t.ls == 0;
+    const bool last_lane = t.ls == Gm::LT - 1;
+    const int wsteps = group_steps<M, L, W>(d, t, qp);
+    float* xf = reinterpret_cast<float*>(xchg + 2);  // carries; xchg[0..1] exchange V
+
+    // projection targets of every thread at qp[i * NT + gl]
     __syncthreads();
 #pragma unroll
-    for (int i = 0; i < M; ++i) qp[seg * L * M + j0 + i] = xv[i];
+    for (int i = 0; i < M; ++i) {
+        const bool proj = d.P.american && d.j0 + i < d.N - 1;
+        qp[i * NT + t.gl] = proj ? xv[i] : -INFINITY;
+    }
     __syncthreads();
-    if (live) {
-        const double* V = qp + seg * L * M;
-        const uint32_t beg = b.off ? b.off[p] : 0u;
-        const uint32_t end = b.off ? b.off[p + 1] : 1u;
-        for (uint32_t k = beg + ls; k < end; k += L) {
-            const uint32_t o2 = b.off ? b.list[k] : oi;
-            const kwgpu_option q = b.off ? b.opts[o2] : opt;
-            double price = NAN;
-            int st = KWGPU_STATUS_BAD_INPUT;
-            if (solve && finite_pos(q.s) && finite_pos(q.k)) {
-                const double xs = log(q.s / q.k);
-                const int j = bracket(P, b.density, xs);
-                if (j == 0 || j == N) {
-                    st = KWGPU_STATUS_OUT_OF_GRID;
-                } else {
-                    const double x1 = grid_x(P, b.density, j), x0 = grid_x(P, b.density, j - 1);
-                    const double v = ((x1 - xs) * V[j - 1] + (xs - x0) * V[j]) / (x1 - x0);
-                    price = q.k * v;
-                    st = isfinite(price) ? KWGPU_STATUS_OK : KWGPU_STATUS_NONFINITE;
-                }
+
+    for (int step = 0; step < wsteps; ++step) {
+        // residual (I - B) V in FP64: -bl (V_{j-1} - V_j) - bu (V_{j+1} - V_j) + sdiag V_j
+        // (bl = 0 on the first row and bu = 0 on the last, so their outer neighbour
+        // only has to be finite)
+        double vprev = dpp<kWaveShr1>(xv[M - 1]);
+        double vnext = dpp<kWaveShl1>(xv[0]);
+        vprev = first_lane ? xv[0] : vprev;
+        vnext = last_lane ? xv[M - 1] : vnext;
+        if constexpr (W == 2) {
+            // the waves meet between rows 1023 and 1024
+            if (t.wv == 0 && t.lane == 63) xchg[0] = xv[M - 1];
+            if (t.wv == 1 && t.lane == 0) xchg[1] = xv[0];
+            __syncthreads();
+            if (t.wv == 1 && t.lane == 0) vprev = xchg[0];
+            if (t.wv == 0 && t.lane == 63) vnext = xchg[1];
+            __syncthreads();
+        }
+        float f[M];
+        double dm = xv[0] - vprev;
+        float fprev = 0.0f;
+#pragma unroll
+        for (int i = 0; i < M; ++i) {
+            const double dp = (i + 1 < M ? xv[i + 1] : vnext) - xv[i];
+            double r = fma(blv[i], dm, sdiag * xv[i]);
+            r = fma(-buv[i], dp, r);
+            dm = dp;
+            fprev = fmaf(-bt[i], fprev, ap[i] * (float)r);
+            f[i] = fprev;
+        }
+        const float c = f[M - 1];
+#pragma unroll
+        for (int i = M - 2; i >= 0; --i) f[i] = fmaf(-gm[i], f[i + 1], f[i]);
+        const float fin = fwd_carry<L, W>(c, sm, t, first_lane, xf);
+        const float zin = bwd_carry<L, W>(fmaf(R[0], fin, f[0]), sm, t, last_lane, xf);
+        if (step < d.steps) {
+#pragma unroll
+            for (int i = 0; i < M; ++i) {
+                float u = fmaf(R[i], fin, f[i]);
+                u = fmaf(Q[i], zin, u);
+                xv[i] = vmax(xv[i] + (double)u, qp[i * NT + t.gl]);
             }
-            b.prices[o2] = price;
-            if (b.status) b.status[o2] = st;
+        }
+    }
+    interpolate<M, L, W>(b, d, t, xv, qp);
+}
+
+// M rows per lane, L lanes per option (W = 1) or W whole waves per option; WPE is the
+// number of waves per SIMD the register budget is cut for (2: <= 256 registers).
+// With PERSIST the grid is persistent: workgroups take group indices from `queue`
+// until all `ngroups` are done (every workgroup reaches the exit test).
+template <int M, int L, int W, int WPE, bool PERSIST, bool MIXED>
+__global__ __attribute__((amdgpu_flat_work_group_size(64 * W, 64 * W),
+                          amdgpu_waves_per_eu(WPE, WPE)))
+void fd1d_wave(FdBatch b, int* queue, int ngroups) {
+    __shared__ double qp[2 * 64 * W * M];
+    __shared__ double xchg[4];
+    auto run = [&](int g) {
+        if constexpr (MIXED)
+            solve_group_mixed<M, L, W>(b, g, qp, xchg);
+        else
+            solve_group_f64<M, L, W>(b, g, qp, xchg);
+    };
+    if constexpr (!PERSIST) {
+        run(blockIdx.x);
+    } else {
+        __shared__ int next;
+        for (;;) {
+            __syncthreads();
+            if (threadIdx.x == 0) next = atomicAdd(queue, 1);
+            __syncthreads();
+            const int g = next;
+            if (g >= ngroups) break;
+            run(g);
         }
     }
 }
 
-template <int M, int L>
-hipError_t launch_wave(const FdBatch& b, hipStream_t s) {
-    constexpr int G = 64 / L;
-    const int blocks = (b.m + G - 1) / G;
-    hipLaunchKernelGGL((fd1d_wave_f64<M, L>), dim3(blocks), dim3(64), 0, s, b);
+__global__ void zero_counters(int* c, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) c[i] = 0;
+}
+
+template <int M, int L, int W, bool MIXED>
+hipError_t launch_wave(const FdBatch& b, hipStream_t s, int* queue) {
+    constexpr int G = Geo<M, L, W>::G;
+    constexpr int WPE = 2;
+    const int ngroups = (b.m + G - 1) / G;
+    int grid = ngroups;
+    if (queue) {
+        // enough workgroups to fill every SIMD at the register-limited occupancy
+        int dev = 0, cus = 256;
+        if (hipGetDevice(&dev) == hipSuccess)
+            (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        grid = std::min(ngroups, cus * 4 * WPE / W);
+        hipLaunchKernelGGL((fd1d_wave<M, L, W, WPE, true, MIXED>), dim3(grid), dim3(64 * W), 0,
+                           s, b, queue, ngroups);
+    } else {
+        hipLaunchKernelGGL((fd1d_wave<M, L, W, WPE, false, MIXED>), dim3(grid), dim3(64 * W), 0,
+                           s, b, queue, ngroups);
+    }
     return hipGetLastError();
+}
+
+template <bool MIXED>
+hipError_t launch_by_size(const FdBatch& b, int max_x_dim, hipStream_t s, int* queue) {
+    if (max_x_dim <= 64) return launch_wave<4, 16, 1, MIXED>(b, s, queue);
+    if (max_x_dim <= 128) return launch_wave<8, 16, 1, MIXED>(b, s, queue);
+    if (max_x_dim <= 256) return launch_wave<16, 16, 1, MIXED>(b, s, queue);
+    if (max_x_dim <= 512) return launch_wave<16, 32, 1, MIXED>(b, s, queue);
+    if (max_x_dim <= 1024) return launch_wave<16, 64, 1, MIXED>(b, s, queue);
+    return launch_wave<16, 64, 2, MIXED>(b, s, queue);
 }
 
 }  // namespace
 
 bool wave_layout_fits(int x_dim, int precision) {
     (void)precision;
-    return x_dim >= 3 && x_dim <= 1024;
+    return x_dim >= 3 && x_dim <= 2048;
 }
 
-hipError_t launch_fd1d_wave(const FdBatch& b, int max_x_dim, int precision, hipStream_t s) {
+hipError_t launch_fd1d_wave(const FdBatch& b, int max_x_dim, int precision, hipStream_t s,
+                            int* queue) {
     if (b.m <= 0) return hipSuccess;
     if (!wave_layout_fits(max_x_dim, precision)) return hipErrorInvalidValue;
-    if (max_x_dim <= 64) return launch_wave<4, 16>(b, s);
-    if (max_x_dim <= 128) return launch_wave<8, 16>(b, s);
-    if (max_x_dim <= 256) return launch_wave<16, 16>(b, s);
-    if (max_x_dim <= 512) return launch_wave<16, 32>(b, s);
-    return launch_wave<16, 64>(b, s);
+    if (precision == KWGPU_FP32_MIXED) return launch_by_size<true>(b, max_x_dim, s, queue);
+    return launch_by_size<false>(b, max_x_dim, s, queue);
+}
+
+hipError_t launch_zero_counters(int* c, int n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(zero_counters, dim3((n + 63) / 64), dim3(64), 0, s, c, n);
+    return hipGetLastError();
 }
 
 }  // namespace kwgpu

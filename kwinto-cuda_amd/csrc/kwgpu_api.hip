@@ -103,6 +103,7 @@ int wave_bucket(int x) {
     if (x <= 256) return 256;
     if (x <= 512) return 512;
     if (x <= 1024) return 1024;
+    if (x <= 2048) return 2048;
     return 0;
 }
 
@@ -111,9 +112,9 @@ int wave_bucket(int x) {
 // every PDE uses b.x_dim / b.t_dim.
 int run_fd(kwgpu_ctx* c, FdBatch b, const std::vector<int32_t>& pde_x,
            const std::vector<int32_t>& pde_t, int precision, int layout, hipStream_t s) {
-    if (precision != KWGPU_FP64)
-        return fail(c, KWGPU_ERR_UNSUPPORTED, "kwgpu: FP32-mixed precision is not built");
     const bool uniform = pde_x.empty();
+    if (precision == KWGPU_FP32_MIXED && layout == KWGPU_LAYOUT_LANE_SOA)
+        return fail(c, KWGPU_ERR_UNSUPPORTED, "kwgpu: FP32-mixed runs in the wave layout only");
     auto use_wave = [&](int x) {
         if (layout == KWGPU_LAYOUT_LANE_SOA) return false;
         return kwgpu::wave_layout_fits(x, precision);
@@ -122,10 +123,13 @@ int run_fd(kwgpu_ctx* c, FdBatch b, const std::vector<int32_t>& pde_x,
         int mx = b.x_dim;
         if (!uniform) mx = *std::max_element(pde_x.begin(), pde_x.end());
         if (!kwgpu::wave_layout_fits(mx, precision))
-            return fail(c, KWGPU_ERR_UNSUPPORTED, "kwgpu: wave layout supports x_dim <= 1024");
+            return fail(c, KWGPU_ERR_UNSUPPORTED, "kwgpu: wave layout supports x_dim <= 2048");
     }
 
     auto launch_lane = [&](FdBatch bb, int max_x) -> int {
+        if (precision != KWGPU_FP64)
+            return fail(c, KWGPU_ERR_UNSUPPORTED,
+                        "kwgpu: FP32-mixed supports x_dim <= 2048 (wave layout)");
         const size_t need = kwgpu::lane_workspace_doubles(bb.m, max_x) * sizeof(double);
         if (hipError_t e = ensure(c->ws, need); e != hipSuccess)
             return fail(c, KWGPU_ERR_NOMEM, "kwgpu: lane workspace allocation failed");

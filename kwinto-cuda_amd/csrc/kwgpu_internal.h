@@ -139,7 +139,11 @@ __device__ __forceinline__ int bracket(const PdeParams& p, double density, doubl
 // Kernel launchers (defined in the .hip files, called by the C-ABI layer).
 namespace kwgpu {
 // Wave layout, FP64 / FP32-mixed. Returns hipErrorInvalidValue when x_dim does not fit.
-hipError_t launch_fd1d_wave(const FdBatch& b, int max_x_dim, int precision, hipStream_t s);
+// queue: null for one workgroup per PDE group, or a zeroed device counter for a
+// persistent grid that takes groups from it.
+hipError_t launch_fd1d_wave(const FdBatch& b, int max_x_dim, int precision, hipStream_t s,
+                            int* queue = nullptr);
+hipError_t launch_zero_counters(int* c, int n, hipStream_t s);
 bool wave_layout_fits(int x_dim, int precision);
 // Lane layout (any grid size).
 hipError_t launch_fd1d_lane(const FdBatch& b, hipStream_t s);

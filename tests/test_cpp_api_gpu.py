@@ -20,8 +20,9 @@ def _write_reference_csv(path, opts, want):
     with open(path, "w") as f:
         f.write("expiry,spot,strike,volatility,interest_rate,dividend_rate,parity,exercise,price\n")
         for a, p in zip(opts, want):
-            f.write(f"{a['t']!r},{a['s']!r},{a['k']!r},{a['z']!r},{a['r']!r},{a['q']!r},"
-                    f"{'c' if a['w'] > 0 else 'p'},{'a' if a['e'] else 'e'},{p!r}\n")
+            t, s, k, z, r, q = (repr(float(a[c])) for c in ("t", "s", "k", "z", "r", "q"))
+            f.write(f"{t},{s},{k},{z},{r},{q},{'c' if a['w'] > 0 else 'p'},"
+                    f"{'a' if a['e'] else 'e'},{float(p)!r}\n")
 
 
 def test_reference_test_cases_through_cpp_api(tmp_path):

@@ -48,6 +48,15 @@ def parse_args():
     ap.add_argument("--cpu-sample", type=int, default=1024,
                     help="options of the workload the CPU baseline prices (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--uneven", type=int, default=65536,
+                    help="options of the uneven-grid batch measured beside the headline "
+                         "(0 = skip)")
+    ap.add_argument("--surrogate", action="store_true",
+                    help="also train the premium ensemble on FD labels and time "
+                         "surrogate-first serving")
+    ap.add_argument("--sur-train", type=int, default=2_000_000)
+    ap.add_argument("--sur-serve", type=int, default=1 << 20)
+    ap.add_argument("--sur-epochs", type=int, default=30)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="after the timed region, run one device copy of this size "
                          "(calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -79,6 +88,67 @@ def cpu_baseline(opts, sample, threads):
                       f"{threads} threads, one job per PDE ({dt:.2f} s)"}
 
 
+def bench_uneven(ctx, torch, stream, n, steps, log):
+    """Uneven-grid batch: x-points per option in {256,512,1024,2048}, t-points
+    max(32, ceil(365 T)), sigma U[0.05,0.8] with a 2% low-vol/long-maturity slice
+    (seed 2); autotuned per bucket, captured in a hipGraph and replayed."""
+    import kwgpu
+    from kwgpu import workloads as W
+    o, xd, td = W.uneven(n, 2)
+    dev = torch.device("cuda:0")
+    d_o = torch.from_numpy(o.view(np.uint8).copy()).to(dev)
+    d_x = torch.from_numpy(xd).to(dev)
+    d_t = torch.from_numpy(td).to(dev)
+    d_p = torch.empty(n, dtype=torch.float64, device=dev)
+    d_s = torch.empty(n, dtype=torch.int32, device=dev)
+    torch.cuda.synchronize()
+    flops = float(np.sum(10.0 * xd.astype(np.float64) * (td - 1)))
+    out = {"workload": f"{n} options, x-points in {{256,512,1024,2048}}, "
+                       f"t-points max(32, ceil(365 T)), seed 2",
+           "point_steps": float(np.sum(xd.astype(np.float64) * (td - 1)))}
+    prices = {}
+    for name, prec in (("fp64", kwgpu.FP64), ("fp32_mixed", kwgpu.FP32_MIXED)):
+        prm = kwgpu.params(precision=prec, x_dims=d_x.data_ptr(), t_dims=d_t.data_ptr())
+        tune = ctx.autotune(d_o.data_ptr(), n, d_p.data_ptr(), d_s.data_ptr(), prm)
+        chosen = {b["bucket"]: b["candidates"][b["chosen"]] for b in tune["buckets"]}
+        ctx.fd1d_price_device(d_o.data_ptr(), n, d_p.data_ptr(), d_s.data_ptr(), prm,
+                              stream=stream.cuda_stream)  # plans and captures the graph
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(steps):
+            ctx.fd1d_price_device(d_o.data_ptr(), n, d_p.data_ptr(), d_s.data_ptr(), prm,
+                                  stream=stream.cuda_stream)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / steps
+        if not bool((d_s == 0).all()):
+            raise RuntimeError("uneven batch: failed options")
+        prices[name] = d_p.cpu().numpy().copy()
+        peak = FP64_PEAK_TFLOPS if prec == kwgpu.FP64 else 157.3
+        out[name] = {"options_per_s": n / (ms * 1e-3), "ms_per_step": ms,
+                     "tflops": flops / (ms * 1e-3) / 1e12,
+                     "roofline_frac": flops / (ms * 1e-3) / 1e12 / peak,
+                     "autotuned": {str(k): v for k, v in chosen.items()}}
+        log(f"bench: uneven {name} {ms:.3f} ms/step")
+    d = np.abs(prices["fp32_mixed"] - prices["fp64"])
+    out["fp32_mixed"]["max_abs_err_vs_fp64_fd"] = float(d.max())
+    out["fp32_mixed"]["rms_abs_err_vs_fp64_fd"] = float(np.sqrt(np.mean(d * d)))
+    return out
+
+
+def bench_surrogate(ctx, args, log):
+    from kwgpu import pipeline
+    from kwgpu import workloads as W
+    sur, train = pipeline.train_and_evaluate(ctx, n_train=args.sur_train,
+                                             epochs=args.sur_epochs, log=log)
+    serve = pipeline.serve(ctx, sur, W.serving(args.sur_serve, 4), train["spread_bound"],
+                           log=log)
+    sur.close()
+    return {"training": train, "serving": serve}
+
+
 def main():
     args = parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -98,7 +168,10 @@ def main():
     d_prices = torch.empty(N_OPTIONS, dtype=torch.float64, device=dev)
     d_status = torch.empty(N_OPTIONS, dtype=torch.int32, device=dev)
     ctx = kwgpu.Context(0)
-    stream = torch.cuda.current_stream(dev)
+    # a stream of our own (default priority): kernels launched on it are bracketed by the
+    # HIP events recorded on it (the legacy null stream would not order with them)
+    stream = torch.cuda.Stream(device=dev)
+    torch.cuda.synchronize()
 
     def run_form(precision, steps, warmup):
         """Times `steps` launches of one form; returns (ms/step wall, kernel ms list, prices)."""
@@ -157,6 +230,18 @@ def main():
     traffic = read_traffic()
     value = N_OPTIONS / (ms_per_step * 1e-3)
 
+    log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
+    extra = {}
+    if args.uneven > 0 and args.layout != "lane":
+        extra["uneven"] = bench_uneven(ctx, torch, stream, args.uneven, max(1, args.steps // 2),
+                                       log)
+    if args.surrogate:
+        extra["surrogate"] = bench_surrogate(ctx, args, log)
+        forms["surrogate_first"] = {
+            "options_per_s": extra["surrogate"]["serving"]["surrogate_first_options_per_s"],
+            "max_abs_err_vs_fp64_fd": extra["surrogate"]["serving"]["max_abs_err_vs_fp64_fd"],
+            "workload": f"{args.sur_serve} serving options (seed 4), 1024x1024 FD fallback"}
+
     cpu = None
     if args.cpu_sample > 0:
         threads = min(args.cpu_threads, os.cpu_count() or 1)
@@ -200,6 +285,7 @@ def main():
             "algorithmic_flops_per_launch": flops,
         },
         "cpu_baseline": cpu,
+        "extra": extra,
     }
     print(json.dumps(out))
 

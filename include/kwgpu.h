@@ -64,8 +64,9 @@ enum {
 /* Arithmetic of the FD march. */
 enum {
     KWGPU_FP64 = 0,      /* everything in FP64: matches the reference to <= 1e-8            */
-    KWGPU_FP32_MIXED = 1 /* FP64 grid, coefficients, LU pivots, V accumulation and
-                            interpolation; FP32 RHS, substitutions and correction        */
+    KWGPU_FP32_MIXED = 1 /* FP64 grid, coefficients, LU pivots, step residual, V
+                            accumulation, projection and interpolation; FP32
+                            substitutions, carries and fix-ups: <= 1e-4 vs FP64        */
 };
 
 /* Where one option's system lives during the march. */
@@ -140,11 +141,107 @@ int kwgpu_bs_price_device(kwgpu_ctx* ctx, const kwgpu_option* d_opts, size_t n,
 int kwgpu_fd1d_bs_price(kwgpu_ctx* ctx, const kwgpu_option* opts, size_t n,
                         const kwgpu_fd1d_params* params, double* prices, int32_t* status);
 
+/*
+ * Autotuner (no counterpart in the reference). For every grid-size bucket of this
+ * batch (x-points rounded up to a power of two, 64 .. 2048), times on a sample of the
+ * bucket's options each wave-layout variant that holds it (rows per lane x lanes per
+ * option: 4, 2 or 1 options per wave, or two waves per option), with a static and with
+ * a persistent grid, and in FP64 also the lane layout; keeps the fastest per (bucket,
+ * precision) in the context, which later calls use. `report` (may be NULL) receives a
+ * JSON object with every timing. Device pointers as for kwgpu_fd1d_price_device (the
+ * sampled options' prices are written); synchronous.
+ */
+int kwgpu_fd1d_autotune(kwgpu_ctx* ctx, const kwgpu_option* d_opts, size_t n,
+                        const kwgpu_fd1d_params* params, double* d_prices, int32_t* d_status,
+                        char* report, size_t report_len);
+
+/* Saves / loads the context's autotuned choices (text, one "bucket precision variant
+ * persistent" line each). */
+int kwgpu_tune_save(kwgpu_ctx* ctx, const char* path);
+int kwgpu_tune_load(kwgpu_ctx* ctx, const char* path);
+
 /* Waits for all work queued on the context's stream. */
 int kwgpu_synchronize(kwgpu_ctx* ctx);
 
 /* The context's stream (a hipStream_t). */
 void* kwgpu_stream(kwgpu_ctx* ctx);
+
+/* ---------------------------------------------------------------------------------
+ * Learned early-exercise premium (no counterpart in the reference; it sits in front
+ * of the FD pricer above and uses it as its source of labels and as its fallback).
+ *
+ * An ensemble of MLPs 6 -> 256 -> 256 -> 256 -> 256 -> 1 with SiLU predicts
+ * y = asinh(premium / (K * 1e-5)) / 4, where premium = FD American price - Black-Scholes
+ * European price, from features (log(S/K), T, sigma, r, q, w) normalised to the box
+ * K/S in [0.4, 2], T in [1/52, 5], sigma in [0.1, 0.6], r in [0, 0.1], q in [0, 0.08].
+ * The three 256x256 layers run on BF16 MFMA with FP32 accumulation (forward and
+ * backward); the 6-wide input layer and the 1-wide output layer run in FP32; Adam
+ * keeps FP32 master weights.
+ * --------------------------------------------------------------------------------- */
+typedef struct kwgpu_surrogate kwgpu_surrogate;
+
+/* Parameters per member, in this order: W1[256][6], b1[256], W2[256][256], b2[256],
+ * W3[256][256], b3[256], W4[256][256], b4[256], W5[256], b5[1]. */
+#define KWGPU_SURROGATE_PARAMS 199425
+
+typedef struct kwgpu_surrogate_train_params {
+    int32_t epochs;     /* passes over the training set                         */
+    int32_t batch;      /* samples per Adam step                                 */
+    float lr;           /* initial learning rate (cosine decay to lr_final)      */
+    float lr_final;
+    float beta1, beta2, eps;
+    uint64_t seed;      /* shuffling                                            */
+} kwgpu_surrogate_train_params;
+
+void kwgpu_surrogate_train_params_default(kwgpu_surrogate_train_params* p);
+
+/* Creates `members` networks with seeded (He-uniform) initial weights. */
+int kwgpu_surrogate_create(kwgpu_ctx* ctx, int32_t members, uint64_t seed,
+                           kwgpu_surrogate** out);
+void kwgpu_surrogate_destroy(kwgpu_surrogate* s);
+
+/* Trains every member on (options, FD premium) pairs held on the device; member m
+ * shuffles with seed + m. loss_out (host, may be NULL) receives epochs x members
+ * mean training losses. Synchronous. */
+int kwgpu_surrogate_train(kwgpu_ctx* ctx, kwgpu_surrogate* s, const kwgpu_option* d_opts,
+                          const double* d_premium, size_t n,
+                          const kwgpu_surrogate_train_params* p, float* loss_out);
+
+/* Ensemble inference: premium (price units, mean over members), spread (max - min over
+ * members, price units) and whether the option lies in the training box. Device
+ * pointers, asynchronous on `stream` (NULL = the context's stream). */
+int kwgpu_surrogate_predict(kwgpu_ctx* ctx, kwgpu_surrogate* s, const kwgpu_option* d_opts,
+                            size_t n, double* d_premium, double* d_spread, uint8_t* d_in_box,
+                            void* stream);
+
+/* Surrogate-first pricing: options in the box whose ensemble spread is <= spread_max
+ * are priced as BS + premium; the rest go to the FD pricer with `fd`. d_route[i] is 0
+ * for the surrogate, 1 for FD. *n_fd receives the number routed to FD. Device
+ * pointers; synchronous (the FD subset is compacted on the device). */
+int kwgpu_surrogate_price(kwgpu_ctx* ctx, kwgpu_surrogate* s, const kwgpu_option* d_opts,
+                          size_t n, double spread_max, const kwgpu_fd1d_params* fd,
+                          double* d_prices, int32_t* d_route, size_t* n_fd);
+
+/* Weights of one member as FP32 (KWGPU_SURROGATE_PARAMS values, host memory). */
+int kwgpu_surrogate_get_params(kwgpu_ctx* ctx, const kwgpu_surrogate* s, int32_t member,
+                               float* host_params);
+int kwgpu_surrogate_set_params(kwgpu_ctx* ctx, kwgpu_surrogate* s, int32_t member,
+                               const float* host_params);
+
+/* One forward/backward pass of `member` on normalised features (n x 6, FP32) and
+ * scaled targets (n, FP32), both host memory: returns the mean squared error and its
+ * gradient with respect to every parameter (host, KWGPU_SURROGATE_PARAMS values). */
+int kwgpu_surrogate_loss_grad(kwgpu_ctx* ctx, kwgpu_surrogate* s, int32_t member,
+                              const float* host_features, const float* host_targets, size_t n,
+                              float* host_loss, float* host_grad);
+
+/* Normalised features (n x 6) of options, host memory, as the network sees them. */
+int kwgpu_surrogate_features(kwgpu_ctx* ctx, const kwgpu_option* host_opts, size_t n,
+                             float* host_features);
+
+/* Saves / loads an ensemble (a small binary file: magic, members, FP32 weights). */
+int kwgpu_surrogate_save(kwgpu_ctx* ctx, const kwgpu_surrogate* s, const char* path);
+int kwgpu_surrogate_load(kwgpu_ctx* ctx, const char* path, kwgpu_surrogate** out);
 
 #ifdef __cplusplus
 }

@@ -5,23 +5,6 @@
 
 namespace kwgpu {
 
-// BlackScholes_Pricer::priceOne, src/Pricer/kwBlackScholes.cpp:27-50:
-// NaN for American options; cdfNormal(x) = (1 + erf(x / sqrt 2)) / 2 (src/Math/kwMath.cpp:9-13).
-#pragma clang fp contract(off)
-__device__ __forceinline__ double cdf_normal(double x) {
-    return 0.5 * (1 + erf(x / 1.4142135623730951));
-}
-
-__device__ __forceinline__ double bs_one(const kwgpu_option& a) {
-    if (a.e) return NAN;
-    const double k = a.k, q = a.q, r = a.r, s = a.s, t = a.t, z = a.z;
-    const double w = (double)a.w;
-    const double zt = z * sqrt(t);
-    const double d1 = 1 / zt * (log(s / k) + (r - q + 0.5 * z * z) * t);
-    const double d2 = d1 - zt;
-    return w * (s * cdf_normal(w * d1) * exp(-q * t) - k * cdf_normal(w * d2) * exp(-r * t));
-}
-#pragma clang fp contract(on)
 
 namespace {
 

@@ -23,14 +23,19 @@
 // row_shl and readlane backward) whose multipliers depend only on B and so are
 // precomputed; per-row fix-up coefficients fold the carries back in.
 //
-// FP64 march: 7 instructions per point and step (2 forward, 1 backward, 3 fix-up and
-// correction, 1 projection) plus the scans amortised over M rows. The state, the
-// factors and the forward fix-up live in VGPRs; the backward fix-up and the payoff
-// live in LDS (one ds_read_b128 per row and step) so that a wave stays within 256
-// registers and two waves share each SIMD.
+// The factors are kept in LDU form: unit-lower l_j = bl_j / d_{j-1}, pivots d_j and
+// the scaled upper u_j = bu_j / d_{j+1}. With g = L^{-1} V (g_j = V_j - l_j g_{j-1}) and
+// w_j = g_j - u_j w_{j+1}, B^{-1} V = D^{-1} w, so with a_j = 1 / (theta d_j)
+//     V'_j = a_j w_j - ((1-theta)/theta) V_j              (= a_j w_j - V_j at theta 1/2).
 //
-// FP32-mixed march: the same factorisation applied to the step's increment,
-//     V' - V = (1/theta) B^{-1} (I - B) V,
+// FP64 march: 6 instructions per point and step at theta 1/2 (1 forward, 1 backward,
+// 2 carry fix-ups, 1 correction, 1 projection) plus the scans amortised over M rows.
+// The state, a, l, u and the forward fix-up live in VGPRs; the backward fix-up and the
+// payoff live in LDS (one ds_read_b128 per row and step) so that a wave stays within
+// 256 registers and two waves share each SIMD.
+//
+// FP32-mixed march: the same factors applied to the step's increment,
+//     V' - V = (1/theta) B^{-1} (I - B) V = a (L D U)... = a_j w_j with g, w taken of (I - B) V,
 // with the residual (I - B) V assembled in FP64 from FP64 differences of V (so its
 // rounding error is relative to the increment, not to V), the substitutions, carries
 // and fix-ups in FP32 on FP32 copies of the factors, and V accumulated, projected and
@@ -158,13 +163,14 @@ __device__ __forceinline__ Pde pde_of(const FdBatch& b, int group, const Lanes& 
 }
 
 // ---- set-up: grid, operator rows, payoff, LU factors, carry coefficients (FP64) ----
-// qp: LDS scratch of at least 4 * NT doubles. On return xv holds the payoff, ap/bt/gm
-// the scaled LU factors, R/Q the carry fix-ups, blv/buv the off-diagonals of B, and
-// sm the scan multipliers.
+// qp: LDS scratch of at least 4 * NT doubles. On return xv holds the payoff, ap the
+// scaled inverse pivots 1/(theta d), lo/up the unit-lower and scaled upper factors
+// l_j = bl_j / d_{j-1}, u_j = bu_j / d_{j+1}, R/Q the carry fix-ups, blv/buv the
+// off-diagonals of B, and sm the scan multipliers.
 template <int M, int L, int W>
 __device__ __forceinline__ void setup(const FdBatch& b, const Pde& d, const Lanes& t, double* qp,
-                                      double (&xv)[M], double (&ap)[M], double (&bt)[M],
-                                      double (&gm)[M], double (&R)[M], double (&Q)[M],
+                                      double (&xv)[M], double (&ap)[M], double (&lo)[M],
+                                      double (&up)[M], double (&R)[M], double (&Q)[M],
                                       double (&blv)[M], double (&buv)[M], ScanMul<double>& sm) {
     using Gm = Geo<M, L, W>;
     constexpr int NT = Gm::NT, LT = Gm::LT;
@@ -228,6 +234,7 @@ __device__ __forceinline__ void setup(const FdBatch& b, const Pde& d, const Lane
         const double is = s > 0.0 ? 1.0 / s : 1.0;
         s00 = n00 * is; s01 = n01 * is; s10 = n10 * is; s11 = n11 * is;
     }
+    double dv[M];
     {
         double dprev = t.ls == 0 ? 1.0 : (s00 + s01) / (s10 + s11);
         // Thomas pivots of the reference (gam_j = bu_{j-1}/bet, bet = b_j - bl_j gam_j)
@@ -236,28 +243,38 @@ __device__ __forceinline__ void setup(const FdBatch& b, const Pde& d, const Lane
         for (int i = 0; i < M; ++i) {
             const double gam = bup / dprev;
             const double dd = bv[i] - blv[i] * gam;
+            lo[i] = blv[i] / dprev;  // bl = 0 on row 0, whatever the entering pivot
+            dv[i] = dd;
             ap[i] = inv_th / dd;
-            bt[i] = blv[i] / dd;
-            gm[i] = buv[i] / dd;
             dprev = dd;
             bup = buv[i];
         }
     }
+    // u_j = bu_j / d_{j+1}: the last row of a lane needs the next lane's first pivot
+    // (bu = 0 on the last real row and on padding rows, so the segment end needs none)
+    __syncthreads();
+    qp[gl] = dv[0];
+    __syncthreads();
+    {
+        const double dnext = t.ls + 1 < LT ? qp[gl + 1] : 1.0;
+#pragma unroll
+        for (int i = 0; i < M; ++i) up[i] = buv[i] / (i + 1 < M ? dv[i + 1] : dnext);
+    }
 
-    // Carry propagation coefficients: f_j = fhat_j + Pc_j f_in,
-    // z_j = zhat_j + R_j f_in + Q_j z_in.
+    // Carry propagation coefficients: g_j = ghat_j + Pc_j g_in,
+    // w_j = what_j + R_j g_in + Q_j w_in.
     double Pl, Ql;
     {
         double Pc[M];
-        Pc[0] = -bt[0];
+        Pc[0] = -lo[0];
 #pragma unroll
-        for (int i = 1; i < M; ++i) Pc[i] = -bt[i] * Pc[i - 1];
+        for (int i = 1; i < M; ++i) Pc[i] = -lo[i] * Pc[i - 1];
         R[M - 1] = Pc[M - 1];
 #pragma unroll
-        for (int i = M - 2; i >= 0; --i) R[i] = Pc[i] - gm[i] * R[i + 1];
-        Q[M - 1] = -gm[M - 1];
+        for (int i = M - 2; i >= 0; --i) R[i] = Pc[i] - up[i] * R[i + 1];
+        Q[M - 1] = -up[M - 1];
 #pragma unroll
-        for (int i = M - 2; i >= 0; --i) Q[i] = -gm[i] * Q[i + 1];
+        for (int i = M - 2; i >= 0; --i) Q[i] = -up[i] * Q[i + 1];
         Pl = Pc[M - 1];
         Ql = Q[0];
     }
@@ -409,6 +426,39 @@ __device__ __forceinline__ void interpolate(const FdBatch& b, const Pde& d, cons
 }
 
 // ---- FP64 march (src/Math/kwFd1d.cpp:67-133) -----------------------------------------
+// One step of the scheme on the state xv; HALF: theta = 1/2, where the correction is
+// a single FMA a_j w_j - V_j.
+template <int M, int L, int W, bool HALF>
+__device__ __forceinline__ void march_f64(double (&xv)[M], const double (&ap)[M],
+                                          const double (&lo)[M], const double (&up)[M],
+                                          const double (&R)[M], const double2* qpv,
+                                          const ScanMul<double>& sm, const Lanes& t,
+                                          bool first_lane, bool last_lane, double* xchg,
+                                          int wsteps, int steps, double c_th) {
+    constexpr int NT = Geo<M, L, W>::NT;
+    for (int step = 0; step < wsteps; ++step) {
+        double g[M];
+        g[0] = xv[0];
+#pragma unroll
+        for (int i = 1; i < M; ++i) g[i] = fma(-lo[i], g[i - 1], xv[i]);
+        const double c = g[M - 1];
+#pragma unroll
+        for (int i = M - 2; i >= 0; --i) g[i] = fma(-up[i], g[i + 1], g[i]);
+        const double gin = fwd_carry<L, W>(c, sm, t, first_lane, xchg);
+        const double win = bwd_carry<L, W>(fma(R[0], gin, g[0]), sm, t, last_lane, xchg);
+        if (step < steps) {
+#pragma unroll
+            for (int i = 0; i < M; ++i) {
+                const double2 qv = qpv[i * NT + t.gl];
+                double w = fma(R[i], gin, g[i]);
+                w = fma(qv.x, win, w);
+                const double v = HALF ? fma(ap[i], w, -xv[i]) : fma(ap[i], w, -c_th * xv[i]);
+                xv[i] = vmax(v, qv.y);
+            }
+        }
+    }
+}
+
 template <int M, int L, int W>
 __device__ __forceinline__ void solve_group_f64(const FdBatch& b, int group, double* qp,
                                                 double* xchg) {
@@ -417,9 +467,9 @@ __device__ __forceinline__ void solve_group_f64(const FdBatch& b, int group, dou
     const Lanes t = lanes_of<M, L, W>(threadIdx.x);
     const Pde d = pde_of<M, L, W>(b, group, t);
     const double c_th = (1.0 - b.theta) / b.theta;
-    double xv[M], ap[M], bt[M], gm[M], R[M], Q[M], blv[M], buv[M];
+    double xv[M], ap[M], lo[M], up[M], R[M], Q[M], blv[M], buv[M];
     ScanMul<double> sm;
-    setup<M, L, W>(b, d, t, qp, xv, ap, bt, gm, R, Q, blv, buv, sm);
+    setup<M, L, W>(b, d, t, qp, xv, ap, lo, up, R, Q, blv, buv, sm);
     const bool first_lane = t.ls == 0;
     const bool last_lane = t.ls == Gm::LT - 1;
     const int wsteps = group_steps<M, L, W>(d, t, qp);
@@ -433,27 +483,12 @@ __device__ __forceinline__ void solve_group_f64(const FdBatch& b, int group, dou
         qpv[i * NT + t.gl] = make_double2(Q[i], proj ? xv[i] : -INFINITY);
     }
     __syncthreads();
-
-    for (int step = 0; step < wsteps; ++step) {
-        double f[M];
-        f[0] = ap[0] * xv[0];
-#pragma unroll
-        for (int i = 1; i < M; ++i) f[i] = fma(-bt[i], f[i - 1], ap[i] * xv[i]);
-        const double c = f[M - 1];
-#pragma unroll
-        for (int i = M - 2; i >= 0; --i) f[i] = fma(-gm[i], f[i + 1], f[i]);
-        const double fin = fwd_carry<L, W>(c, sm, t, first_lane, xchg);
-        const double zin = bwd_carry<L, W>(fma(R[0], fin, f[0]), sm, t, last_lane, xchg);
-        if (step < d.steps) {
-#pragma unroll
-            for (int i = 0; i < M; ++i) {
-                const double2 qv = qpv[i * NT + t.gl];
-                double u = fma(R[i], fin, f[i]);
-                u = fma(qv.x, zin, u);
-                xv[i] = vmax(fma(-c_th, xv[i], u), qv.y);
-            }
-        }
-    }
+    if (c_th == 1.0)
+        march_f64<M, L, W, true>(xv, ap, lo, up, R, qpv, sm, t, first_lane, last_lane, xchg,
+                                 wsteps, d.steps, c_th);
+    else
+        march_f64<M, L, W, false>(xv, ap, lo, up, R, qpv, sm, t, first_lane, last_lane, xchg,
+                                  wsteps, d.steps, c_th);
     interpolate<M, L, W>(b, d, t, xv, qp);
 }
 
@@ -466,17 +501,17 @@ __device__ __forceinline__ void solve_group_mixed(const FdBatch& b, int group, d
     const Lanes t = lanes_of<M, L, W>(threadIdx.x);
     const Pde d = pde_of<M, L, W>(b, group, t);
     double xv[M], blv[M], buv[M];
-    float ap[M], bt[M], gm[M], R[M], Q[M];
+    float ap[M], lo[M], up[M], R[M], Q[M];
     ScanMul<float> sm;
     {
-        double ap64[M], bt64[M], gm64[M], R64[M], Q64[M];
+        double ap64[M], lo64[M], up64[M], R64[M], Q64[M];
         ScanMul<double> s64;
-        setup<M, L, W>(b, d, t, qp, xv, ap64, bt64, gm64, R64, Q64, blv, buv, s64);
+        setup<M, L, W>(b, d, t, qp, xv, ap64, lo64, up64, R64, Q64, blv, buv, s64);
 #pragma unroll
         for (int i = 0; i < M; ++i) {
             ap[i] = (float)ap64[i];
-            bt[i] = (float)bt64[i];
-            gm[i] = (float)gm64[i];
+            lo[i] = (float)lo64[i];
+            up[i] = (float)up64[i];
             R[i] = (float)R64[i];
             Q[i] = (float)Q64[i];
         }
@@ -520,29 +555,29 @@ __device__ __forceinline__ void solve_group_mixed(const FdBatch& b, int group, d
             if (t.wv == 0 && t.lane == 63) vnext = xchg[1];
             __syncthreads();
         }
-        float f[M];
+        float g[M];
         double dm = xv[0] - vprev;
-        float fprev = 0.0f;
+        float gprev = 0.0f;
 #pragma unroll
         for (int i = 0; i < M; ++i) {
             const double dp = (i + 1 < M ? xv[i + 1] : vnext) - xv[i];
             double r = fma(blv[i], dm, sdiag * xv[i]);
             r = fma(-buv[i], dp, r);
             dm = dp;
-            fprev = fmaf(-bt[i], fprev, ap[i] * (float)r);
-            f[i] = fprev;
+            gprev = fmaf(-lo[i], gprev, (float)r);
+            g[i] = gprev;
         }
-        const float c = f[M - 1];
+        const float c = g[M - 1];
 #pragma unroll
-        for (int i = M - 2; i >= 0; --i) f[i] = fmaf(-gm[i], f[i + 1], f[i]);
-        const float fin = fwd_carry<L, W>(c, sm, t, first_lane, xf);
-        const float zin = bwd_carry<L, W>(fmaf(R[0], fin, f[0]), sm, t, last_lane, xf);
+        for (int i = M - 2; i >= 0; --i) g[i] = fmaf(-up[i], g[i + 1], g[i]);
+        const float gin = fwd_carry<L, W>(c, sm, t, first_lane, xf);
+        const float win = bwd_carry<L, W>(fmaf(R[0], gin, g[0]), sm, t, last_lane, xf);
         if (step < d.steps) {
 #pragma unroll
             for (int i = 0; i < M; ++i) {
-                float u = fmaf(R[i], fin, f[i]);
-                u = fmaf(Q[i], zin, u);
-                xv[i] = vmax(xv[i] + (double)u, qp[i * NT + t.gl]);
+                float w = fmaf(R[i], gin, g[i]);
+                w = fmaf(Q[i], win, w);
+                xv[i] = vmax(fma((double)ap[i], (double)w, xv[i]), qp[i * NT + t.gl]);
             }
         }
     }
@@ -607,28 +642,60 @@ hipError_t launch_wave(const FdBatch& b, hipStream_t s, int* queue) {
 }
 
 template <bool MIXED>
-hipError_t launch_by_size(const FdBatch& b, int max_x_dim, hipStream_t s, int* queue) {
-    if (max_x_dim <= 64) return launch_wave<4, 16, 1, MIXED>(b, s, queue);
-    if (max_x_dim <= 128) return launch_wave<8, 16, 1, MIXED>(b, s, queue);
-    if (max_x_dim <= 256) return launch_wave<16, 16, 1, MIXED>(b, s, queue);
-    if (max_x_dim <= 512) return launch_wave<16, 32, 1, MIXED>(b, s, queue);
-    if (max_x_dim <= 1024) return launch_wave<16, 64, 1, MIXED>(b, s, queue);
-    return launch_wave<16, 64, 2, MIXED>(b, s, queue);
+hipError_t launch_variant(const FdBatch& b, int v, hipStream_t s, int* queue) {
+    switch (v) {
+        case 0: return launch_wave<4, 16, 1, MIXED>(b, s, queue);
+        case 1: return launch_wave<8, 16, 1, MIXED>(b, s, queue);
+        case 2: return launch_wave<16, 16, 1, MIXED>(b, s, queue);
+        case 3: return launch_wave<4, 32, 1, MIXED>(b, s, queue);
+        case 4: return launch_wave<8, 32, 1, MIXED>(b, s, queue);
+        case 5: return launch_wave<16, 32, 1, MIXED>(b, s, queue);
+        case 6: return launch_wave<4, 64, 1, MIXED>(b, s, queue);
+        case 7: return launch_wave<8, 64, 1, MIXED>(b, s, queue);
+        case 8: return launch_wave<16, 64, 1, MIXED>(b, s, queue);
+        case 9: return launch_wave<16, 64, 2, MIXED>(b, s, queue);
+        default: return hipErrorInvalidValue;
+    }
 }
 
 }  // namespace
+
+// rows per lane M, lanes per option L, waves per option W (index = variant id)
+static const WaveVariant kVariants[] = {{4, 16, 1}, {8, 16, 1}, {16, 16, 1}, {4, 32, 1},
+                                        {8, 32, 1}, {16, 32, 1}, {4, 64, 1}, {8, 64, 1},
+                                        {16, 64, 1}, {16, 64, 2}};
+
+int wave_variant_count() { return (int)(sizeof(kVariants) / sizeof(kVariants[0])); }
+WaveVariant wave_variant(int v) { return kVariants[v]; }
+
+// Fewest lanes per option at 16 rows per lane (4 or 8 for the smallest grids).
+int default_wave_variant(int max_x_dim) {
+    if (max_x_dim <= 64) return 0;
+    if (max_x_dim <= 128) return 1;
+    if (max_x_dim <= 256) return 2;
+    if (max_x_dim <= 512) return 5;
+    if (max_x_dim <= 1024) return 8;
+    if (max_x_dim <= 2048) return 9;
+    return -1;
+}
 
 bool wave_layout_fits(int x_dim, int precision) {
     (void)precision;
     return x_dim >= 3 && x_dim <= 2048;
 }
 
+hipError_t launch_fd1d_wave_variant(const FdBatch& b, int variant, int precision, hipStream_t s,
+                                    int* queue) {
+    if (b.m <= 0) return hipSuccess;
+    if (variant < 0 || variant >= wave_variant_count()) return hipErrorInvalidValue;
+    if (precision == KWGPU_FP32_MIXED) return launch_variant<true>(b, variant, s, queue);
+    return launch_variant<false>(b, variant, s, queue);
+}
+
 hipError_t launch_fd1d_wave(const FdBatch& b, int max_x_dim, int precision, hipStream_t s,
                             int* queue) {
-    if (b.m <= 0) return hipSuccess;
     if (!wave_layout_fits(max_x_dim, precision)) return hipErrorInvalidValue;
-    if (precision == KWGPU_FP32_MIXED) return launch_by_size<true>(b, max_x_dim, s, queue);
-    return launch_by_size<false>(b, max_x_dim, s, queue);
+    return launch_fd1d_wave_variant(b, default_wave_variant(max_x_dim), precision, s, queue);
 }
 
 hipError_t launch_zero_counters(int* c, int n, hipStream_t s) {

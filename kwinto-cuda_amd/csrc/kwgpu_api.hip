@@ -5,6 +5,8 @@
 // error text) and dispatches the FD kernels by layout and grid size.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
+#include <map>
 #include <cstdio>
 #include <cstring>
 #include <numeric>
@@ -23,12 +25,40 @@ struct DevBuf {
 
 }  // namespace
 
+// Kernel choice for one grid-size bucket: a wave-layout variant (or -1 for the lane
+// layout) and whether its grid is persistent.
+struct Choice {
+    int variant = -1;
+    bool persist = true;
+};
+
+// One launch of an uneven batch: PDEs order[begin, begin + count) of one bucket.
+struct Bucket {
+    int cap;         // grid-size bucket: power of two >= x_dim (64 .. 2048), or kLaneCap
+    Choice ch;
+    size_t begin, count;
+    int max_x;
+};
+
+constexpr int kLaneCap = 1 << 30;
+constexpr int kSide = 4;  // side streams an uneven batch forks onto
+
 struct kwgpu_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     std::string err;
-    DevBuf opts, prices, status, rep, off, list, order, xd, td, ws;
+    DevBuf opts, prices, status, rep, off, list, order, xd, td, ws, counters;
     DevBuf opts2, prices2, status2;  // second leg of FD1D-BS
+    // autotuned choices per (bucket, precision)
+    std::map<std::pair<int, int>, Choice> tuned;
+    // uneven batches: side streams, fork/join events, the cached plan and its graph
+    hipStream_t side[kSide] = {};
+    hipEvent_t ev_fork = nullptr, ev_join[kSide] = {};
+    bool plan_valid = false;
+    uint64_t plan_key = 0;
+    std::vector<Bucket> plan;
+    hipGraphExec_t graph = nullptr;
+    bool use_graphs = true;
 };
 
 namespace {
@@ -96,96 +126,206 @@ int check_dims(kwgpu_ctx* c, const int32_t* xd, const int32_t* td, size_t n) {
     return KWGPU_OK;
 }
 
-// Wave-layout instantiation a grid size maps to (0 = does not fit).
-int wave_bucket(int x) {
-    if (x <= 64) return 64;
-    if (x <= 128) return 128;
-    if (x <= 256) return 256;
-    if (x <= 512) return 512;
-    if (x <= 1024) return 1024;
-    if (x <= 2048) return 2048;
-    return 0;
+// Grid-size bucket of an x_dim: the power of two >= x_dim (at least 64), or kLaneCap
+// for grids the wave layout does not hold.
+int bucket_of(int x, int precision, int layout) {
+    if (layout == KWGPU_LAYOUT_LANE_SOA || !kwgpu::wave_layout_fits(x, precision)) return kLaneCap;
+    int c = 64;
+    while (c < x) c <<= 1;
+    return c;
+}
+
+Choice choice_for(const kwgpu_ctx* c, int cap, int precision, int layout) {
+    Choice ch;
+    if (cap == kLaneCap) return ch;  // lane layout
+    auto it = c->tuned.find({cap, precision});
+    if (it != c->tuned.end() && !(layout == KWGPU_LAYOUT_WAVE_REG && it->second.variant < 0))
+        return it->second;
+    ch.variant = kwgpu::default_wave_variant(cap);
+    ch.persist = true;
+    return ch;
+}
+
+uint64_t fnv(uint64_t h, const void* p, size_t n) {
+    const unsigned char* b = (const unsigned char*)p;
+    for (size_t i = 0; i < n; ++i) h = (h ^ b[i]) * 1099511628211ull;
+    return h;
+}
+
+int lane_launch(kwgpu_ctx* c, FdBatch bb, int max_x, int precision, hipStream_t s, double* ws) {
+    if (precision != KWGPU_FP64)
+        return fail(c, KWGPU_ERR_UNSUPPORTED, "kwgpu: FP32-mixed runs in the wave layout only");
+    bb.ws = ws;
+    bb.ws_stride = bb.m;
+    bb.ws_rows = max_x;
+    if (hipError_t e = kwgpu::launch_fd1d_lane(bb, s); e != hipSuccess)
+        return hip_fail(c, e, "kwgpu: lane kernel launch");
+    return KWGPU_OK;
+}
+
+// Launches the buckets of an uneven batch: bucket i on side stream i % kSide (forked
+// from and joined back into s), persistent buckets taking work from counters[i].
+int launch_buckets(kwgpu_ctx* c, const FdBatch& b, const std::vector<Bucket>& plan, int precision,
+                   hipStream_t s) {
+    hipError_t e;
+    if ((e = kwgpu::launch_zero_counters((int*)c->counters.p, (int)plan.size(), s)))
+        return hip_fail(c, e, "kwgpu: counter reset");
+    if ((e = hipEventRecord(c->ev_fork, s))) return hip_fail(c, e, "kwgpu: fork");
+    size_t ws_off = 0;
+    bool used[kSide] = {};
+    for (size_t i = 0; i < plan.size(); ++i) {
+        const Bucket& k = plan[i];
+        hipStream_t ss = c->side[i % kSide];
+        if (!used[i % kSide] && (e = hipStreamWaitEvent(ss, c->ev_fork, 0)))
+            return hip_fail(c, e, "kwgpu: fork wait");
+        used[i % kSide] = true;
+        FdBatch bb = b;
+        bb.order = (const uint32_t*)c->order.p + k.begin;
+        bb.m = (int)k.count;
+        if (k.ch.variant < 0) {
+            double* ws = (double*)c->ws.p + ws_off;
+            ws_off += kwgpu::lane_workspace_doubles(bb.m, k.max_x);
+            if (int rc = lane_launch(c, bb, k.max_x, precision, ss, ws)) return rc;
+        } else if ((e = kwgpu::launch_fd1d_wave_variant(bb, k.ch.variant, precision, ss,
+                                                        k.ch.persist ? (int*)c->counters.p + i
+                                                                     : nullptr))) {
+            return hip_fail(c, e, "kwgpu: wave kernel launch");
+        }
+    }
+    for (int k = 0; k < kSide; ++k) {
+        if (!used[k]) continue;
+        if ((e = hipEventRecord(c->ev_join[k], c->side[k])) || (e = hipStreamWaitEvent(s, c->ev_join[k], 0)))
+            return hip_fail(c, e, "kwgpu: join");
+    }
+    return KWGPU_OK;
+}
+
+int ensure_side_streams(kwgpu_ctx* c) {
+    if (c->ev_fork) return KWGPU_OK;
+    hipError_t e;
+    for (int k = 0; k < kSide; ++k)
+        if ((e = hipStreamCreateWithFlags(&c->side[k], hipStreamNonBlocking)) ||
+            (e = hipEventCreateWithFlags(&c->ev_join[k], hipEventDisableTiming)))
+            return hip_fail(c, e, "kwgpu: side streams");
+    if ((e = hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming)))
+        return hip_fail(c, e, "kwgpu: side streams");
+    return KWGPU_OK;
+}
+
+void drop_graph(kwgpu_ctx* c) {
+    if (c->graph) (void)hipGraphExecDestroy(c->graph);
+    c->graph = nullptr;
+    c->plan_valid = false;
 }
 
 // Runs the FD solve of `b` (device buffers already set) on stream s. pde_x / pde_t are
-// host copies of each PDE's grid sizes, used to group PDEs into launches; empty means
-// every PDE uses b.x_dim / b.t_dim.
+// host copies of each PDE's grid sizes; empty means every PDE uses b.x_dim / b.t_dim.
+//
+// Uneven batches: PDEs are grouped by grid-size bucket, each bucket sorted by t-points
+// (descending) so that co-packed segments march for about the same number of steps and
+// a persistent grid takes the longest first; each bucket's kernel (autotuned variant)
+// runs on its own side stream. The whole launch sequence is captured once into a
+// hipGraph and replayed while the batch (sizes, buffers, parameters) is unchanged.
 int run_fd(kwgpu_ctx* c, FdBatch b, const std::vector<int32_t>& pde_x,
            const std::vector<int32_t>& pde_t, int precision, int layout, hipStream_t s) {
     const bool uniform = pde_x.empty();
     if (precision == KWGPU_FP32_MIXED && layout == KWGPU_LAYOUT_LANE_SOA)
         return fail(c, KWGPU_ERR_UNSUPPORTED, "kwgpu: FP32-mixed runs in the wave layout only");
-    auto use_wave = [&](int x) {
-        if (layout == KWGPU_LAYOUT_LANE_SOA) return false;
-        return kwgpu::wave_layout_fits(x, precision);
-    };
-    if (layout == KWGPU_LAYOUT_WAVE_REG) {
-        int mx = b.x_dim;
-        if (!uniform) mx = *std::max_element(pde_x.begin(), pde_x.end());
-        if (!kwgpu::wave_layout_fits(mx, precision))
-            return fail(c, KWGPU_ERR_UNSUPPORTED, "kwgpu: wave layout supports x_dim <= 2048");
-    }
-
-    auto launch_lane = [&](FdBatch bb, int max_x) -> int {
-        if (precision != KWGPU_FP64)
-            return fail(c, KWGPU_ERR_UNSUPPORTED,
-                        "kwgpu: FP32-mixed supports x_dim <= 2048 (wave layout)");
-        const size_t need = kwgpu::lane_workspace_doubles(bb.m, max_x) * sizeof(double);
-        if (hipError_t e = ensure(c->ws, need); e != hipSuccess)
-            return fail(c, KWGPU_ERR_NOMEM, "kwgpu: lane workspace allocation failed");
-        bb.ws = (double*)c->ws.p;
-        bb.ws_stride = bb.m;
-        bb.ws_rows = max_x;
-        if (hipError_t e = kwgpu::launch_fd1d_lane(bb, s); e != hipSuccess)
-            return hip_fail(c, e, "kwgpu: lane kernel launch");
-        return KWGPU_OK;
-    };
+    const int mx = uniform ? b.x_dim : *std::max_element(pde_x.begin(), pde_x.end());
+    if (layout == KWGPU_LAYOUT_WAVE_REG && !kwgpu::wave_layout_fits(mx, precision))
+        return fail(c, KWGPU_ERR_UNSUPPORTED, "kwgpu: wave layout supports x_dim <= 2048");
+    if (precision == KWGPU_FP32_MIXED && !kwgpu::wave_layout_fits(mx, precision))
+        return fail(c, KWGPU_ERR_UNSUPPORTED, "kwgpu: FP32-mixed supports x_dim <= 2048");
 
     if (uniform) {
-        if (use_wave(b.x_dim)) {
-            if (hipError_t e = kwgpu::launch_fd1d_wave(b, b.x_dim, precision, s); e != hipSuccess)
-                return hip_fail(c, e, "kwgpu: wave kernel launch");
-            return KWGPU_OK;
+        const int cap = bucket_of(b.x_dim, precision, layout);
+        Choice ch = choice_for(c, cap, precision, layout);
+        if (ch.variant < 0) {
+            const size_t need = kwgpu::lane_workspace_doubles(b.m, b.x_dim) * sizeof(double);
+            if (c->ws.cap < need) drop_graph(c);  // a captured graph may hold the old buffer
+            if (ensure(c->ws, need) != hipSuccess)
+                return fail(c, KWGPU_ERR_NOMEM, "kwgpu: lane workspace allocation failed");
+            return lane_launch(c, b, b.x_dim, precision, s, (double*)c->ws.p);
         }
-        return launch_lane(b, b.x_dim);
+        // uniform batches launch directly: one kernel, nothing to fork or reorder
+        if (hipError_t e = kwgpu::launch_fd1d_wave_variant(b, ch.variant, precision, s); e != hipSuccess)
+            return hip_fail(c, e, "kwgpu: wave kernel launch");
+        return KWGPU_OK;
     }
 
-    // Uneven grids: one launch per wave instantiation (PDEs sorted by t-points so the
-    // segments packed into one wave march for about the same number of steps), and one
-    // lane-layout launch for grids the wave layout does not cover.
     const int m = b.m;
+    std::vector<int> cap(m);
+    for (int p = 0; p < m; ++p) cap[p] = bucket_of(pde_x[p], precision, layout);
     std::vector<uint32_t> ord(m);
     std::iota(ord.begin(), ord.end(), 0u);
-    auto key = [&](uint32_t p) {
-        const int wb = use_wave(pde_x[p]) ? wave_bucket(pde_x[p]) : 1 << 30;
-        return std::make_tuple(wb, -pde_t[p], p);
-    };
-    std::sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t z) { return key(a) < key(z); });
-    if (hipError_t e = ensure(c->order, sizeof(uint32_t) * m); e != hipSuccess)
-        return fail(c, KWGPU_ERR_NOMEM, "kwgpu: order buffer allocation failed");
-    if (hipError_t e = hipMemcpyAsync(c->order.p, ord.data(), sizeof(uint32_t) * m,
-                                      hipMemcpyHostToDevice, s);
-        e != hipSuccess)
-        return hip_fail(c, e, "kwgpu: order upload");
-    size_t i = 0;
-    while (i < (size_t)m) {
-        const int wb = std::get<0>(key(ord[i]));
+    std::sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t z) {
+        return std::make_tuple(cap[a], -pde_t[a], a) < std::make_tuple(cap[z], -pde_t[z], z);
+    });
+    std::vector<Bucket> plan;
+    for (size_t i = 0; i < (size_t)m;) {
         size_t j = i;
-        int mx = 0;
-        while (j < (size_t)m && std::get<0>(key(ord[j])) == wb) mx = std::max(mx, pde_x[ord[j++]]);
-        FdBatch bb = b;
-        bb.order = (const uint32_t*)c->order.p + i;
-        bb.m = (int)(j - i);
-        int rc;
-        if (wb != 1 << 30) {
-            hipError_t e = kwgpu::launch_fd1d_wave(bb, mx, precision, s);
-            rc = e == hipSuccess ? KWGPU_OK : hip_fail(c, e, "kwgpu: wave kernel launch");
-        } else {
-            rc = launch_lane(bb, mx);
-        }
-        if (rc) return rc;
+        int bx = 0;
+        while (j < (size_t)m && cap[ord[j]] == cap[ord[i]]) bx = std::max(bx, pde_x[ord[j++]]);
+        plan.push_back(Bucket{cap[ord[i]], choice_for(c, cap[ord[i]], precision, layout), i, j - i, bx});
         i = j;
     }
+    // the batch's identity: sizes, buffers, parameters and kernel choices
+    uint64_t key = 1469598103934665603ull;
+    key = fnv(key, pde_x.data(), pde_x.size() * sizeof(int32_t));
+    key = fnv(key, pde_t.data(), pde_t.size() * sizeof(int32_t));
+    key = fnv(key, &b, sizeof(b));
+    key = fnv(key, &precision, sizeof(precision));
+    for (const Bucket& k : plan) key = fnv(key, &k, sizeof(k));
+
+    // ... and the context buffers a captured graph would reference
+    auto with_buffers = [&](uint64_t k) {
+        const void* bufs[3] = {c->order.p, c->counters.p, c->ws.p};
+        return fnv(k, bufs, sizeof(bufs));
+    };
+    if (int rc = ensure_side_streams(c)) return rc;
+    const bool rebuild = !c->plan_valid || c->plan_key != with_buffers(key);
+    if (rebuild) {
+        drop_graph(c);
+        size_t ws = 0;
+        for (const Bucket& k : plan)
+            if (k.ch.variant < 0) ws += kwgpu::lane_workspace_doubles((int)k.count, k.max_x);
+        if (ensure(c->order, sizeof(uint32_t) * m) != hipSuccess ||
+            ensure(c->counters, sizeof(int) * plan.size()) != hipSuccess ||
+            (ws && ensure(c->ws, ws * sizeof(double)) != hipSuccess))
+            return fail(c, KWGPU_ERR_NOMEM, "kwgpu: uneven batch allocation failed");
+        key = with_buffers(key);
+        hipError_t e;
+        if ((e = hipMemcpyAsync(c->order.p, ord.data(), sizeof(uint32_t) * m, hipMemcpyHostToDevice, s)) ||
+            (e = hipStreamSynchronize(s)))
+            return hip_fail(c, e, "kwgpu: order upload");
+        c->plan = plan;
+        c->plan_key = key;
+        c->plan_valid = true;
+    }
+    if (!c->use_graphs) return launch_buckets(c, b, c->plan, precision, s);
+    if (!c->graph) {
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess) {
+            const int rc = launch_buckets(c, b, c->plan, precision, s);
+            hipError_t e2 = hipStreamEndCapture(s, &g);
+            if (rc) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc;
+            }
+            e = e2;
+        }
+        if (e == hipSuccess) e = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+        if (g) (void)hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            // streams that cannot be captured (e.g. the legacy null stream): launch directly
+            (void)hipGetLastError();
+            c->graph = nullptr;
+            return launch_buckets(c, b, c->plan, precision, s);
+        }
+    }
+    if (hipError_t e = hipGraphLaunch(c->graph, s); e != hipSuccess)
+        return hip_fail(c, e, "kwgpu: graph launch");
     return KWGPU_OK;
 }
 
@@ -315,6 +455,12 @@ int collect(kwgpu_ctx* c, const kwgpu_option* opts, size_t n, const kwgpu_fd1d_p
 
 }  // namespace
 
+namespace kwgpu {
+int ctx_fail(kwgpu_ctx* c, int code, const std::string& msg) { return fail(c, code, msg); }
+hipStream_t ctx_stream(kwgpu_ctx* c) { return c ? c->stream : nullptr; }
+int ctx_device(kwgpu_ctx* c) { return c ? c->device : 0; }
+}  // namespace kwgpu
+
 extern "C" {
 
 void kwgpu_fd1d_params_default(kwgpu_fd1d_params* p) {
@@ -338,6 +484,7 @@ int kwgpu_create(kwgpu_ctx** out, int device) {
     if (hipSetDevice(device) != hipSuccess) return KWGPU_ERR_HIP;
     auto* c = new kwgpu_ctx();
     c->device = device;
+    if (const char* g = std::getenv("KWGPU_NO_GRAPH")) c->use_graphs = g[0] == '0';
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
         return KWGPU_ERR_HIP;
@@ -350,8 +497,14 @@ void kwgpu_destroy(kwgpu_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
+    drop_graph(c);
+    for (int k = 0; k < kSide; ++k) {
+        if (c->side[k]) (void)hipStreamDestroy(c->side[k]);
+        if (c->ev_join[k]) (void)hipEventDestroy(c->ev_join[k]);
+    }
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
     for (DevBuf* b : {&c->opts, &c->prices, &c->status, &c->rep, &c->off, &c->list, &c->order,
-                      &c->xd, &c->td, &c->ws, &c->opts2, &c->prices2, &c->status2})
+                      &c->xd, &c->td, &c->ws, &c->counters, &c->opts2, &c->prices2, &c->status2})
         free_buf(*b);
     (void)hipStreamDestroy(c->stream);
     delete c;
@@ -497,6 +650,169 @@ int kwgpu_fd1d_bs_price(kwgpu_ctx* c, const kwgpu_option* opts, size_t n,
         if (st[i] == KWGPU_STATUS_OK && st2[i] != KWGPU_STATUS_OK) st[i] = st2[i];
     if (status) std::memcpy(status, st.data(), n * sizeof(int32_t));
     return collect(c, opts, n, p, st, prefix);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------------
+// Autotuner
+// ---------------------------------------------------------------------------------
+extern "C" {
+
+int kwgpu_fd1d_autotune(kwgpu_ctx* c, const kwgpu_option* d_opts, size_t n,
+                        const kwgpu_fd1d_params* p, double* d_prices, int32_t* d_status,
+                        char* report, size_t report_len) {
+    if (!c) return KWGPU_ERR_INVALID;
+    c->err.clear();
+    if (int rc = check_params(c, p)) return rc;
+    if (!d_opts || !d_prices || n == 0 || n > 0x7fffffffu)
+        return fail(c, KWGPU_ERR_INVALID, "kwgpu_fd1d_autotune: bad argument");
+    if (hipError_t e = hipSetDevice(c->device); e != hipSuccess) return hip_fail(c, e, "hipSetDevice");
+    hipStream_t s = c->stream;
+    std::vector<int32_t> hx(n, p->x_dim), ht(n, p->t_dim);
+    hipError_t e;
+    if ((p->x_dims && (e = hipMemcpy(hx.data(), p->x_dims, n * sizeof(int32_t), hipMemcpyDeviceToHost))) ||
+        (p->t_dims && (e = hipMemcpy(ht.data(), p->t_dims, n * sizeof(int32_t), hipMemcpyDeviceToHost))))
+        return hip_fail(c, e, "kwgpu_fd1d_autotune: sizes");
+    if (int rc = check_dims(c, hx.data(), ht.data(), n)) return rc;
+    const int prec = p->precision;
+    std::map<int, std::vector<uint32_t>> buckets;
+    for (size_t i = 0; i < n; ++i)
+        buckets[bucket_of(hx[i], prec, KWGPU_LAYOUT_AUTO)].push_back((uint32_t)i);
+
+    FdBatch b{};
+    b.opts = d_opts;
+    b.x_dim = p->x_dim;
+    b.t_dim = p->t_dim;
+    b.x_dims = p->x_dims;
+    b.t_dims = p->t_dims;
+    b.density = p->density;
+    b.scale = p->scale;
+    b.theta = p->theta;
+    b.prices = d_prices;
+    b.status = d_status;
+    DevBuf sample, cnt, ws;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    std::string js = "{\"precision\": \"" + std::string(prec == KWGPU_FP64 ? "fp64" : "fp32-mixed") +
+                     "\", \"buckets\": [";
+    int rc = KWGPU_OK;
+    if ((e = hipEventCreate(&e0)) || (e = hipEventCreate(&e1))) rc = hip_fail(c, e, "events");
+    bool first_bucket = true;
+    for (auto& [cap, ids] : buckets) {
+        if (rc) break;
+        if (cap == kLaneCap) continue;  // nothing to choose: only the lane layout holds it
+        // a strided sample in longest-first order
+        const size_t S = std::min<size_t>(ids.size(), 8192);
+        std::vector<uint32_t> smp;
+        for (size_t k = 0; k < S; ++k) smp.push_back(ids[k * ids.size() / S]);
+        std::sort(smp.begin(), smp.end(), [&](uint32_t a, uint32_t z) { return ht[a] > ht[z]; });
+        int bx = 0;
+        for (uint32_t i : smp) bx = std::max(bx, hx[i]);
+        if (ensure(sample, S * sizeof(uint32_t)) || ensure(cnt, sizeof(int)) ||
+            (e = hipMemcpy(sample.p, smp.data(), S * sizeof(uint32_t), hipMemcpyHostToDevice))) {
+            rc = fail(c, KWGPU_ERR_NOMEM, "kwgpu_fd1d_autotune: sample upload");
+            break;
+        }
+        FdBatch bb = b;
+        bb.order = (const uint32_t*)sample.p;
+        bb.m = (int)S;
+        struct Cand {
+            Choice ch;
+            float ms;
+        };
+        std::vector<Cand> cands;
+        for (int v = 0; v < kwgpu::wave_variant_count(); ++v)
+            if (kwgpu::wave_variant(v).capacity() == cap)
+                for (bool per : {false, true}) cands.push_back({Choice{v, per}, 0.f});
+        if (prec == KWGPU_FP64) cands.push_back({Choice{-1, false}, 0.f});
+        for (auto& cd : cands) {
+            auto run = [&]() -> int {
+                if (cd.ch.variant < 0) {
+                    if (ensure(ws, kwgpu::lane_workspace_doubles(bb.m, bx) * sizeof(double)))
+                        return fail(c, KWGPU_ERR_NOMEM, "kwgpu_fd1d_autotune: workspace");
+                    return lane_launch(c, bb, bx, prec, s, (double*)ws.p);
+                }
+                int* q = nullptr;
+                if (cd.ch.persist) {
+                    q = (int*)cnt.p;
+                    if (hipError_t e2 = kwgpu::launch_zero_counters(q, 1, s)) return hip_fail(c, e2, "reset");
+                }
+                if (hipError_t e2 = kwgpu::launch_fd1d_wave_variant(bb, cd.ch.variant, prec, s, q))
+                    return hip_fail(c, e2, "kwgpu_fd1d_autotune: launch");
+                return KWGPU_OK;
+            };
+            if ((rc = run())) break;  // warm-up
+            if ((e = hipEventRecord(e0, s))) { rc = hip_fail(c, e, "event"); break; }
+            for (int r = 0; r < 2 && !rc; ++r) rc = run();
+            if (rc) break;
+            if ((e = hipEventRecord(e1, s)) || (e = hipEventSynchronize(e1)) ||
+                (e = hipEventElapsedTime(&cd.ms, e0, e1))) {
+                rc = hip_fail(c, e, "event");
+                break;
+            }
+            cd.ms /= 2;
+        }
+        if (rc) break;
+        const Cand* best = &cands[0];
+        for (const auto& cd : cands)
+            if (cd.ms < best->ms) best = &cd;
+        c->tuned[{cap, prec}] = best->ch;
+        char buf[256];
+        js += first_bucket ? "" : ", ";
+        first_bucket = false;
+        std::snprintf(buf, sizeof(buf), "{\"bucket\": %d, \"options\": %zu, \"sampled\": %zu, \"candidates\": [",
+                      cap, ids.size(), S);
+        js += buf;
+        for (size_t k = 0; k < cands.size(); ++k) {
+            const auto& cd = cands[k];
+            if (cd.ch.variant < 0) {
+                std::snprintf(buf, sizeof(buf), "%s{\"layout\": \"lane\", \"ms\": %.4f}", k ? ", " : "", cd.ms);
+            } else {
+                const auto wv = kwgpu::wave_variant(cd.ch.variant);
+                std::snprintf(buf, sizeof(buf),
+                              "%s{\"layout\": \"wave\", \"rows_per_lane\": %d, \"lanes\": %d, \"waves\": %d, "
+                              "\"persistent\": %s, \"ms\": %.4f}",
+                              k ? ", " : "", wv.M, wv.L * wv.W, wv.W, cd.ch.persist ? "true" : "false", cd.ms);
+            }
+            js += buf;
+        }
+        std::snprintf(buf, sizeof(buf), "], \"chosen\": %d}", (int)(best - &cands[0]));
+        js += buf;
+    }
+    js += "]}";
+    (void)hipStreamSynchronize(s);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    free_buf(sample);
+    free_buf(cnt);
+    free_buf(ws);
+    drop_graph(c);  // choices changed
+    if (report && report_len) std::snprintf(report, report_len, "%s", js.c_str());
+    return rc;
+}
+
+int kwgpu_tune_save(kwgpu_ctx* c, const char* path) {
+    if (!c || !path) return KWGPU_ERR_INVALID;
+    FILE* f = std::fopen(path, "w");
+    if (!f) return fail(c, KWGPU_ERR_INVALID, std::string("cannot write ") + path);
+    for (const auto& [k, ch] : c->tuned)
+        std::fprintf(f, "%d %d %d %d\n", k.first, k.second, ch.variant, ch.persist ? 1 : 0);
+    std::fclose(f);
+    return KWGPU_OK;
+}
+
+int kwgpu_tune_load(kwgpu_ctx* c, const char* path) {
+    if (!c || !path) return KWGPU_ERR_INVALID;
+    FILE* f = std::fopen(path, "r");
+    if (!f) return fail(c, KWGPU_ERR_INVALID, std::string("cannot read ") + path);
+    int cap, prec, v, per;
+    while (std::fscanf(f, "%d %d %d %d", &cap, &prec, &v, &per) == 4) {
+        if (v >= kwgpu::wave_variant_count() || (v >= 0 && kwgpu::wave_variant(v).capacity() != cap)) continue;
+        c->tuned[{cap, prec}] = Choice{v, per != 0};
+    }
+    std::fclose(f);
+    drop_graph(c);
+    return KWGPU_OK;
 }
 
 }  // extern "C"

@@ -132,18 +132,51 @@ __device__ __forceinline__ int bracket(const PdeParams& p, double density, doubl
     while (j < N && grid_x(p, density, j) < xs) ++j;
     return j;
 }
+// BlackScholes_Pricer::priceOne, src/Pricer/kwBlackScholes.cpp:27-50:
+// NaN for American options; cdfNormal(x) = (1 + erf(x / sqrt 2)) / 2 (src/Math/kwMath.cpp:9-13).
+__device__ __forceinline__ double cdf_normal(double x) {
+    return 0.5 * (1 + erf(x / 1.4142135623730951));
+}
+
+__device__ __forceinline__ double bs_one(const kwgpu_option& a) {
+    if (a.e) return NAN;
+    const double k = a.k, q = a.q, r = a.r, s = a.s, t = a.t, z = a.z;
+    const double w = (double)a.w;
+    const double zt = z * sqrt(t);
+    const double d1 = 1 / zt * (log(s / k) + (r - q + 0.5 * z * z) * t);
+    const double d2 = d1 - zt;
+    return w * (s * cdf_normal(w * d1) * exp(-q * t) - k * cdf_normal(w * d2) * exp(-r * t));
+}
 #pragma clang fp contract(on)
 
 }  // namespace kwgpu
 
+#include <string>
+
 // Kernel launchers (defined in the .hip files, called by the C-ABI layer).
 namespace kwgpu {
+// Context internals shared by the C-ABI translation units (kwgpu_api.hip).
+int ctx_fail(kwgpu_ctx* c, int code, const std::string& msg);
+hipStream_t ctx_stream(kwgpu_ctx* c);
+int ctx_device(kwgpu_ctx* c);
+
 // Wave layout, FP64 / FP32-mixed. Returns hipErrorInvalidValue when x_dim does not fit.
 // queue: null for one workgroup per PDE group, or a zeroed device counter for a
 // persistent grid that takes groups from it.
 hipError_t launch_fd1d_wave(const FdBatch& b, int max_x_dim, int precision, hipStream_t s,
                             int* queue = nullptr);
 hipError_t launch_zero_counters(int* c, int n, hipStream_t s);
+// Wave-layout kernel variants: M rows per lane, L lanes per option, W waves per option;
+// one option per segment holds up to M * L * W grid points.
+struct WaveVariant {
+    int M, L, W;
+    int capacity() const { return M * L * W; }
+};
+int wave_variant_count();
+WaveVariant wave_variant(int v);
+int default_wave_variant(int max_x_dim);  // -1 when the grid does not fit
+hipError_t launch_fd1d_wave_variant(const FdBatch& b, int variant, int precision, hipStream_t s,
+                                    int* queue = nullptr);
 bool wave_layout_fits(int x_dim, int precision);
 // Lane layout (any grid size).
 hipError_t launch_fd1d_lane(const FdBatch& b, hipStream_t s);

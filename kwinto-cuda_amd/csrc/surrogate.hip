@@ -1,0 +1,845 @@
+// surrogate.hip -- learned early-exercise premium in front of the FD pricer.
+//
+// An ensemble of MLPs 6 -> 256 -> 256 -> 256 -> 256 -> 1 (SiLU) predicting
+// y = asinh(premium / (K s0)) / c, premium = FD American price - BS European price,
+// s0 = 1e-5, c = 4, from normalised features. The signed-log target makes the network's
+// error relative for large premiums and about K s0 |dy| for small ones, so options whose
+// premium is small can be served to a tight absolute error.
+// The three 256x256 layers run forward and backward on BF16 MFMA
+// (v_mfma_f32_16x16x32_bf16, FP32 accumulation) through one strided GEMM kernel with
+// fused epilogues; the 6-wide input and 1-wide output layers are FP32; Adam keeps FP32
+// master weights and refreshes the BF16 copies after every step. The reference has no
+// counterpart: the labels come from this build's FD pricer, which is also the fallback.
+#include <math.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "kwgpu_internal.h"
+
+namespace kwgpu {
+namespace sur {
+
+typedef __bf16 bf16;
+typedef bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+constexpr int H = 256;  // hidden width
+constexpr int F = 6;    // features
+constexpr int kW1 = 0, kB1 = kW1 + H * F, kW2 = kB1 + H, kB2 = kW2 + H * H, kW3 = kB2 + H,
+              kB3 = kW3 + H * H, kW4 = kB3 + H, kB4 = kW4 + H * H, kW5 = kB4 + H, kB5 = kW5 + H,
+              kP = kB5 + 1;
+static_assert(kP == KWGPU_SURROGATE_PARAMS, "parameter layout");
+constexpr double kS0 = 1e-5;     // premium per unit strike where the target turns logarithmic
+constexpr double kYScale = 4.0;  // target divisor, keeps y within about [-1, 3]
+
+__host__ __device__ inline double premium_of(double y, double k) {
+    return k * kS0 * sinh(kYScale * y);
+}
+
+// Training box, as (centre, half-width) of each raw feature.
+constexpr double kLogMin = -0.69314718055994530942, kLogMax = 0.91629073187415506518;  // K/S 2, 0.4
+constexpr double kBox[F][2] = {
+    {(kLogMin + kLogMax) / 2, (kLogMax - kLogMin) / 2},  // log(S/K)
+    {(1.0 / 52 + 5.0) / 2, (5.0 - 1.0 / 52) / 2},        // T
+    {0.35, 0.25},                                          // sigma
+    {0.05, 0.05},                                          // r
+    {0.04, 0.04},                                          // q
+    {0.0, 1.0},                                            // w
+};
+
+__host__ __device__ inline bool option_features(const kwgpu_option& o, float* f) {
+    const double raw[F] = {log(o.s / o.k), o.t, o.z, o.r, o.q, o.w < 0 ? -1.0 : 1.0};
+    bool in = o.e != 0 && o.s > 0 && o.k > 0;
+    for (int i = 0; i < F; ++i) {
+        const double u = (raw[i] - kBox[i][0]) / kBox[i][1];
+        f[i] = (float)u;
+        in = in && (u >= -1.0 - 1e-12) && (u <= 1.0 + 1e-12);
+    }
+    return in;
+}
+
+__device__ __forceinline__ float sigm(float z) { return 1.0f / (1.0f + __expf(-z)); }
+__device__ __forceinline__ float silu(float z) { return z * sigm(z); }
+__device__ __forceinline__ float dsilu(float z) {
+    const float s = sigm(z);
+    return s * (1.0f + z * (1.0f - s));
+}
+
+// ---- data ------------------------------------------------------------------------
+__global__ void features_kernel(const kwgpu_option* opts, size_t n, const double* premium,
+                                float* feat, float* target, uint8_t* in_box) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const kwgpu_option o = opts[i];
+    float f[F];
+    const bool in = option_features(o, f);
+    for (int k = 0; k < F; ++k) feat[i * F + k] = f[k];
+    if (target) target[i] = (float)(asinh(premium[i] / (o.k * kS0)) / kYScale);
+    if (in_box) in_box[i] = in ? 1 : 0;
+}
+
+__global__ void gather_kernel(const float* feat, const float* target, const uint32_t* idx, int n,
+                              float* bfeat, float* btarget) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t s = idx[i];
+    for (int k = 0; k < F; ++k) bfeat[i * F + k] = feat[(size_t)s * F + k];
+    btarget[i] = target[s];
+}
+
+// ---- layers ----------------------------------------------------------------------
+// input layer in FP32: Z1 = x W1^T + b1, H1 = silu(Z1) in BF16
+__global__ void l1_fwd(const float* x, const float* p, int n, float* Z, bf16* Hb) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n * H) return;
+    const int m = i / H, j = i - m * H;
+    float z = p[kB1 + j];
+#pragma unroll
+    for (int f = 0; f < F; ++f) z = fmaf(p[kW1 + j * F + f], x[m * F + f], z);
+    Z[i] = z;
+    Hb[i] = (bf16)silu(z);
+}
+
+enum { EPI_FWD = 0, EPI_BWD = 1, EPI_WGRAD = 2 };
+
+struct GemmArgs {
+    const bf16* A;  // A(m, k) = A[m * sam + k * sak]
+    long sam, sak;
+    const bf16* B;  // B(k, n) = B[k * sbk + n * sbn]
+    long sbk, sbn;
+    int M, N, K;
+    const float* bias;  // EPI_FWD: bias[n]
+    float* Z;           // EPI_FWD: writes pre-activations; EPI_BWD: reads them
+    bf16* Hout;         // EPI_FWD: silu(Z); EPI_BWD: dZ = C * silu'(Z)
+    float* C;           // EPI_WGRAD: C[m * N + n] += sum over this block's K range
+    int k_split;        // EPI_WGRAD: K range per blockIdx.z
+};
+
+// C(M x N) = A(M x K) B(K x N), 64 x 64 tile per 256-thread workgroup, each wave a
+// 32 x 32 quarter as 2 x 2 MFMA 16x16x32 tiles; K in steps of 32 through LDS.
+template <int EPI>
+__global__ __launch_bounds__(256) void gemm_bf16(GemmArgs g) {
+    constexpr int BM = 64, BN = 64, BK = 32, LD = BK + 8;
+    __shared__ __attribute__((aligned(16))) bf16 As[BM * LD];
+    __shared__ __attribute__((aligned(16))) bf16 Bs[BN * LD];
+    const int t = threadIdx.x, w = t >> 6, l = t & 63;
+    const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+    int kb = 0, ke = g.K;
+    if constexpr (EPI == EPI_WGRAD) {
+        kb = blockIdx.z * g.k_split;
+        ke = min(g.K, kb + g.k_split);
+    }
+    const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
+    floatx4 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+    const int lr = t >> 2, lc = (t & 3) * 8;  // tile row 0..63, k offset 0/8/16/24
+    for (int k0 = kb; k0 < ke; k0 += BK) {
+        {
+            const int m = m0 + lr, k = k0 + lc;
+            bf16x8 v;
+            if (g.sak == 1 && m < g.M && k + 8 <= ke) {
+                v = *reinterpret_cast<const bf16x8*>(g.A + (long)m * g.sam + k);
+            } else {
+#pragma unroll
+                for (int i = 0; i < 8; ++i)
+                    v[i] = (m < g.M && k + i < ke) ? g.A[(long)m * g.sam + (long)(k + i) * g.sak]
+                                                   : (bf16)0.0f;
+            }
+            *reinterpret_cast<bf16x8*>(&As[lr * LD + lc]) = v;
+        }
+        {
+            const int n = n0 + lr, k = k0 + lc;
+            bf16x8 v;
+            if (g.sbk == 1 && n < g.N && k + 8 <= ke) {
+                v = *reinterpret_cast<const bf16x8*>(g.B + (long)n * g.sbn + k);
+            } else {
+#pragma unroll
+                for (int i = 0; i < 8; ++i)
+                    v[i] = (n < g.N && k + i < ke) ? g.B[(long)(k + i) * g.sbk + (long)n * g.sbn]
+                                                   : (bf16)0.0f;
+            }
+            *reinterpret_cast<bf16x8*>(&Bs[lr * LD + lc]) = v;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const bf16x8 a =
+                *reinterpret_cast<const bf16x8*>(&As[(wm + 16 * i + (l & 15)) * LD + 8 * (l >> 4)]);
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                const bf16x8 b = *reinterpret_cast<const bf16x8*>(
+                    &Bs[(wn + 16 * j + (l & 15)) * LD + 8 * (l >> 4)]);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[i][j], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+    // D layout of v_mfma_f32_16x16x32: lane l, element e -> row 4 (l / 16) + e, col l % 16
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int m = m0 + wm + 16 * i + 4 * (l >> 4) + e;
+                const int n = n0 + wn + 16 * j + (l & 15);
+                if (m >= g.M || n >= g.N) continue;
+                const float v = acc[i][j][e];
+                const long o = (long)m * g.N + n;
+                if constexpr (EPI == EPI_FWD) {
+                    const float z = v + g.bias[n];
+                    g.Z[o] = z;
+                    g.Hout[o] = (bf16)silu(z);
+                } else if constexpr (EPI == EPI_BWD) {
+                    g.Hout[o] = (bf16)(v * dsilu(g.Z[o]));
+                } else {
+                    atomicAdd(&g.C[o], v);
+                }
+            }
+}
+
+// output layer: y = b5 + W5 . silu(Z4) (FP32, one wave per sample); with targets,
+// dy = 2 (y - t) / n and the summed squared error.
+__global__ void out_fwd(const float* Z4, const float* p, int n, const float* target, float* y,
+                        float* dy, float* loss_sum) {
+    const int m = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    const int l = threadIdx.x & 63;
+    if (m >= n) return;
+    float s = 0.0f;
+#pragma unroll
+    for (int k = l; k < H; k += 64) s = fmaf(p[kW5 + k], silu(Z4[(long)m * H + k]), s);
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (l == 0) {
+        const float v = s + p[kB5];
+        y[m] = v;
+        if (target) {
+            const float d = v - target[m];
+            dy[m] = 2.0f * d / (float)n;
+            atomicAdd(loss_sum, d * d);
+        }
+    }
+}
+
+// dZ4 = dy W5 silu'(Z4) (BF16 for the next GEMMs)
+__global__ void out_bwd(const float* Z4, const float* p, const float* dy, int n, bf16* dZ) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n * H) return;
+    const int m = i / H, k = i - m * H;
+    dZ[i] = (bf16)(dy[m] * p[kW5 + k] * dsilu(Z4[i]));
+}
+
+// grad[kB? + n] += sum_m dZ[m][n]   and, for the output layer,
+// grad[kW5 + n] += sum_m dy[m] silu(Z4[m][n]), grad[kB5] += sum_m dy[m]
+__global__ void colsum_bf16(const bf16* dZ, int n, float* out) {
+    const int col = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int r0 = blockIdx.y * 4 + (threadIdx.x >> 6);
+    float s = 0.0f;
+    for (int m = r0; m < n; m += 4 * gridDim.y) s += (float)dZ[(long)m * H + col];
+    atomicAdd(&out[col], s);
+}
+__global__ void out_grad(const float* Z4, const float* dy, int n, float* grad) {
+    const int col = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int r0 = blockIdx.y * 4 + (threadIdx.x >> 6);
+    float s = 0.0f, sb = 0.0f;
+    for (int m = r0; m < n; m += 4 * gridDim.y) {
+        s = fmaf(dy[m], silu(Z4[(long)m * H + col]), s);
+        if (col == 0) sb += dy[m];
+    }
+    atomicAdd(&grad[kW5 + col], s);
+    if (col == 0) atomicAdd(&grad[kB5], sb);
+}
+// input layer gradients: grad W1[j][f] += sum_m dZ1[m][j] x[m][f], b1[j] += sum_m dZ1[m][j]
+__global__ void l1_grad(const bf16* dZ1, const float* x, int n, float* grad) {
+    const int col = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int r0 = blockIdx.y * 4 + (threadIdx.x >> 6);
+    float s[F + 1] = {};
+    for (int m = r0; m < n; m += 4 * gridDim.y) {
+        const float d = (float)dZ1[(long)m * H + col];
+#pragma unroll
+        for (int f = 0; f < F; ++f) s[f] = fmaf(d, x[m * F + f], s[f]);
+        s[F] += d;
+    }
+#pragma unroll
+    for (int f = 0; f < F; ++f) atomicAdd(&grad[kW1 + col * F + f], s[f]);
+    atomicAdd(&grad[kB1 + col], s[F]);
+}
+
+__global__ void adam_kernel(float* p, const float* g, float* m1, float* m2, int np, float lr,
+                            float b1, float b2, float eps, float c1, float c2) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    const float gi = g[i];
+    const float a = b1 * m1[i] + (1.0f - b1) * gi;
+    const float v = b2 * m2[i] + (1.0f - b2) * gi * gi;
+    m1[i] = a;
+    m2[i] = v;
+    p[i] -= lr * (a / c1) / (sqrtf(v / c2) + eps);
+}
+
+__global__ void to_bf16(const float* p, bf16* w, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) w[i] = (bf16)p[i];
+}
+
+__global__ void zero_f(float* p, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0f;
+}
+
+// ensemble combine in price units: premium = mean over members, spread = max - min
+__global__ void combine_kernel(const kwgpu_option* opts, const float* y, int members, size_t n,
+                               size_t ld, double* premium, double* spread) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double k = opts[i].k;
+    double s = 0, lo = 1e300, hi = -1e300;
+    for (int m = 0; m < members; ++m) {
+        const double v = premium_of(y[m * ld + i], k);
+        s += v;
+        lo = fmin(lo, v);
+        hi = fmax(hi, v);
+    }
+    premium[i] = s / members;
+    spread[i] = hi - lo;
+}
+
+// ---- serving ---------------------------------------------------------------------
+// Route 0: surrogate (in the box, ensemble spread <= spread_max): price = BS + premium.
+// Route 1: FP32-mixed FD. Route 2: FP64 FD -- options whose far call boundary
+// exp(scale sigma sqrt(T)) would overflow the FP32 increments of the mixed march.
+__global__ void gate_kernel(const kwgpu_option* opts, size_t n, const double* premium,
+                            const double* spread, const uint8_t* in_box, double spread_max,
+                            double fp64_above, double* prices, int32_t* route, int* counts,
+                            uint32_t* idx1, uint32_t* idx2) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    kwgpu_option o = opts[i];
+    int r;
+    double price = NAN;
+    if (in_box[i] && spread[i] <= spread_max && isfinite(premium[i])) {
+        kwgpu_option e = o;
+        e.e = 0;
+        price = bs_one(e) + premium[i];
+        r = isfinite(price) ? 0 : 1;
+    } else {
+        r = 1;
+    }
+    if (r != 0) r = (o.z * sqrt(o.t) > fp64_above) ? 2 : 1;
+    if (r == 1) idx1[atomicAdd(&counts[0], 1)] = (uint32_t)i;
+    if (r == 2) idx2[atomicAdd(&counts[1], 1)] = (uint32_t)i;
+    prices[i] = price;
+    route[i] = r;
+}
+
+__global__ void gather_opts(const kwgpu_option* opts, const uint32_t* idx, int n, kwgpu_option* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = opts[idx[i]];
+}
+
+__global__ void scatter_prices(const double* sub, const int32_t* sub_status, const uint32_t* idx,
+                               int n, double* prices) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) prices[idx[i]] = sub_status[i] == KWGPU_STATUS_OK ? sub[i] : NAN;
+}
+
+}  // namespace sur
+}  // namespace kwgpu
+
+// ---------------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------------
+using namespace kwgpu::sur;
+
+struct kwgpu_surrogate {
+    int members = 0;
+    int device = 0;
+    std::vector<float*> p;             // FP32 master weights per member
+    std::vector<bf16*> wb;             // BF16 W2 | W3 | W4 per member
+    // scratch (grown on demand)
+    size_t cap = 0;
+    float *x = nullptr, *tg = nullptr, *y = nullptr, *dy = nullptr, *Z = nullptr;
+    bf16 *Hb = nullptr, *dZa = nullptr, *dZb = nullptr;
+    float *grad = nullptr, *m1 = nullptr, *m2 = nullptr, *loss = nullptr;
+    uint32_t* idx = nullptr;
+    float* yall = nullptr;
+    size_t yall_cap = 0;
+};
+
+namespace {
+
+#define HIPCHK(expr)                                                              \
+    do {                                                                          \
+        hipError_t e_ = (expr);                                                   \
+        if (e_ != hipSuccess)                                                     \
+            return kwgpu::ctx_fail(ctx, KWGPU_ERR_HIP,                            \
+                                   std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+unsigned nblk(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
+
+void free_scratch(kwgpu_surrogate* s) {
+    for (void* q : {(void*)s->x, (void*)s->tg, (void*)s->y, (void*)s->dy, (void*)s->Z,
+                    (void*)s->Hb, (void*)s->dZa, (void*)s->dZb, (void*)s->idx})
+        if (q) (void)hipFree(q);
+    s->x = s->tg = s->y = s->dy = s->Z = nullptr;
+    s->Hb = s->dZa = s->dZb = nullptr;
+    s->idx = nullptr;
+    s->cap = 0;
+}
+
+// batch scratch for `rows` samples: x, targets, y, dy, Z for 4 layers, H for 4 layers,
+// two dZ buffers
+hipError_t ensure_scratch(kwgpu_surrogate* s, size_t rows) {
+    if (s->cap >= rows) return hipSuccess;
+    free_scratch(s);
+    hipError_t e;
+    if ((e = hipMalloc(&s->x, rows * F * sizeof(float))) ||
+        (e = hipMalloc(&s->tg, rows * sizeof(float))) || (e = hipMalloc(&s->y, rows * sizeof(float))) ||
+        (e = hipMalloc(&s->dy, rows * sizeof(float))) ||
+        (e = hipMalloc(&s->Z, 4 * rows * H * sizeof(float))) ||
+        (e = hipMalloc(&s->Hb, 4 * rows * H * sizeof(bf16))) ||
+        (e = hipMalloc(&s->dZa, rows * H * sizeof(bf16))) ||
+        (e = hipMalloc(&s->dZb, rows * H * sizeof(bf16))) ||
+        (e = hipMalloc(&s->idx, rows * sizeof(uint32_t))))
+        return e;
+    s->cap = rows;
+    return hipSuccess;
+}
+
+GemmArgs gemm_fwd(const bf16* Hin, const bf16* W, int n, const float* bias, float* Z, bf16* Hout) {
+    GemmArgs g{};
+    g.A = Hin; g.sam = H; g.sak = 1;        // H_{l-1}[m][k]
+    g.B = W; g.sbk = 1; g.sbn = H;          // W_l[n][k]
+    g.M = n; g.N = H; g.K = H;
+    g.bias = bias; g.Z = Z; g.Hout = Hout;
+    return g;
+}
+
+hipError_t launch_gemm(int epi, const GemmArgs& g, hipStream_t st, int splits = 1) {
+    dim3 grid(nblk(g.N, 64), nblk(g.M, 64), splits);
+    if (epi == EPI_FWD)
+        hipLaunchKernelGGL(gemm_bf16<EPI_FWD>, grid, dim3(256), 0, st, g);
+    else if (epi == EPI_BWD)
+        hipLaunchKernelGGL(gemm_bf16<EPI_BWD>, grid, dim3(256), 0, st, g);
+    else
+        hipLaunchKernelGGL(gemm_bf16<EPI_WGRAD>, grid, dim3(256), 0, st, g);
+    return hipGetLastError();
+}
+
+// forward of one member on s->x (n rows); y into `yout`; with targets also dy and loss
+hipError_t forward(kwgpu_surrogate* s, int mem, int n, float* yout, const float* targets,
+                   hipStream_t st) {
+    const float* p = s->p[mem];
+    const bf16* wb = s->wb[mem];
+    float* Z = s->Z;
+    bf16* Hb = s->Hb;
+    const size_t L = (size_t)n * H;
+    hipLaunchKernelGGL(l1_fwd, dim3(nblk(L, 256)), dim3(256), 0, st, s->x, p, n, Z, Hb);
+    for (int l = 0; l < 3; ++l) {
+        GemmArgs g = gemm_fwd(Hb + l * L, wb + l * H * H, n, p + kB2 + l * (H * H + H),
+                              Z + (l + 1) * L, Hb + (l + 1) * L);
+        if (hipError_t e = launch_gemm(EPI_FWD, g, st)) return e;
+    }
+    hipLaunchKernelGGL(out_fwd, dim3(nblk(n, 4)), dim3(256), 0, st, Z + 3 * L, p, n, targets, yout,
+                       s->dy, s->loss);
+    return hipGetLastError();
+}
+
+// backward of one member after forward(); gradient accumulated into s->grad (zeroed here)
+hipError_t backward(kwgpu_surrogate* s, int mem, int n, hipStream_t st) {
+    const float* p = s->p[mem];
+    const bf16* wb = s->wb[mem];
+    const size_t L = (size_t)n * H;
+    float* g = s->grad;
+    hipLaunchKernelGGL(zero_f, dim3(nblk(kP, 256)), dim3(256), 0, st, g, (size_t)kP);
+    const int rs = std::max(1, std::min(64, n / 256));  // row splits of the reductions
+    hipLaunchKernelGGL(out_grad, dim3(H / 64, rs), dim3(256), 0, st, s->Z + 3 * L, s->dy, n, g);
+    hipLaunchKernelGGL(out_bwd, dim3(nblk(L, 256)), dim3(256), 0, st, s->Z + 3 * L, p, s->dy, n,
+                       s->dZa);
+    bf16* dz = s->dZa;
+    bf16* dzn = s->dZb;
+    for (int l = 2; l >= 0; --l) {  // hidden layer l+2: weights W_{l+2}, input H_{l+1}
+        const int splits = std::max(1, std::min(32, n / 512));
+        GemmArgs gw{};
+        gw.A = dz; gw.sam = 1; gw.sak = H;               // A(o, m) = dZ[m][o]
+        gw.B = s->Hb + l * L; gw.sbk = H; gw.sbn = 1;    // B(m, i) = H_{l+1}[m][i]
+        gw.M = H; gw.N = H; gw.K = n;
+        gw.C = g + kW2 + l * (H * H + H);
+        gw.k_split = ((n + splits - 1) / splits + 31) / 32 * 32;
+        if (hipError_t e = launch_gemm(EPI_WGRAD, gw, st, splits)) return e;
+        hipLaunchKernelGGL(colsum_bf16, dim3(H / 64, rs), dim3(256), 0, st, dz, n,
+                           g + kB2 + l * (H * H + H));
+        GemmArgs gb{};
+        gb.A = dz; gb.sam = H; gb.sak = 1;                     // A(m, o) = dZ[m][o]
+        gb.B = wb + l * H * H; gb.sbk = H; gb.sbn = 1;         // B(o, i) = W[o][i]
+        gb.M = n; gb.N = H; gb.K = H;
+        gb.Z = s->Z + l * L;                                   // pre-activations of H_{l+1}
+        gb.Hout = dzn;
+        if (hipError_t e = launch_gemm(EPI_BWD, gb, st)) return e;
+        std::swap(dz, dzn);
+    }
+    hipLaunchKernelGGL(l1_grad, dim3(H / 64, rs), dim3(256), 0, st, dz, s->x, n, g);
+    return hipGetLastError();
+}
+
+hipError_t refresh_bf16(kwgpu_surrogate* s, int mem, hipStream_t st) {
+    for (int l = 0; l < 3; ++l)
+        hipLaunchKernelGGL(to_bf16, dim3(nblk(H * H, 256)), dim3(256), 0, st,
+                           s->p[mem] + kW2 + l * (H * H + H), s->wb[mem] + l * H * H, H * H);
+    return hipGetLastError();
+}
+
+void init_params(std::vector<float>& w, uint64_t seed) {
+    std::mt19937_64 rng(seed);
+    w.assign(kP, 0.0f);
+    auto fill = [&](int off, int rows, int fan_in) {
+        const float a = std::sqrt(6.0f / (float)fan_in);  // He-uniform for SiLU
+        std::uniform_real_distribution<float> u(-a, a);
+        for (int i = 0; i < rows * fan_in; ++i) w[off + i] = u(rng);
+    };
+    fill(kW1, H, F);
+    fill(kW2, H, H);
+    fill(kW3, H, H);
+    fill(kW4, H, H);
+    fill(kW5, 1, H);
+    for (int i = 0; i < H; ++i) w[kW5 + i] *= 0.1f;
+}
+
+}  // namespace
+
+extern "C" {
+
+void kwgpu_surrogate_train_params_default(kwgpu_surrogate_train_params* p) {
+    if (!p) return;
+    p->epochs = 30;
+    p->batch = 8192;
+    p->lr = 2e-3f;
+    p->lr_final = 2e-5f;
+    p->beta1 = 0.9f;
+    p->beta2 = 0.999f;
+    p->eps = 1e-8f;
+    p->seed = 7;
+}
+
+int kwgpu_surrogate_create(kwgpu_ctx* ctx, int32_t members, uint64_t seed, kwgpu_surrogate** out) {
+    if (!ctx || !out || members < 1 || members > 16)
+        return kwgpu::ctx_fail(ctx, KWGPU_ERR_INVALID, "kwgpu_surrogate_create: bad argument");
+    *out = nullptr;
+    HIPCHK(hipSetDevice(kwgpu::ctx_device(ctx)));
+    auto* s = new kwgpu_surrogate();
+    s->members = members;
+    s->device = kwgpu::ctx_device(ctx);
+    std::vector<float> w;
+    hipStream_t st = kwgpu::ctx_stream(ctx);
+    for (int m = 0; m < members; ++m) {
+        float* p = nullptr;
+        bf16* b = nullptr;
+        if (hipMalloc(&p, kP * sizeof(float)) || hipMalloc(&b, 3 * H * H * sizeof(bf16))) {
+            kwgpu_surrogate_destroy(s);
+            return kwgpu::ctx_fail(ctx, KWGPU_ERR_NOMEM, "kwgpu_surrogate_create: allocation");
+        }
+        s->p.push_back(p);
+        s->wb.push_back(b);
+        init_params(w, seed * 1000003ull + (uint64_t)m);
+        if (hipMemcpy(p, w.data(), kP * sizeof(float), hipMemcpyHostToDevice) ||
+            refresh_bf16(s, m, st) || hipStreamSynchronize(st)) {
+            kwgpu_surrogate_destroy(s);
+            return kwgpu::ctx_fail(ctx, KWGPU_ERR_HIP, "kwgpu_surrogate_create: upload");
+        }
+    }
+    if (hipMalloc(&s->grad, kP * sizeof(float)) || hipMalloc(&s->m1, kP * sizeof(float)) ||
+        hipMalloc(&s->m2, kP * sizeof(float)) || hipMalloc(&s->loss, sizeof(float))) {
+        kwgpu_surrogate_destroy(s);
+        return kwgpu::ctx_fail(ctx, KWGPU_ERR_NOMEM, "kwgpu_surrogate_create: allocation");
+    }
+    *out = s;
+    return KWGPU_OK;
+}
+
+void kwgpu_surrogate_destroy(kwgpu_surrogate* s) {
+    if (!s) return;
+    (void)hipSetDevice(s->device);
+    (void)hipDeviceSynchronize();
+    for (auto* p : s->p) (void)hipFree(p);
+    for (auto* b : s->wb) (void)hipFree(b);
+    free_scratch(s);
+    for (void* q : {(void*)s->grad, (void*)s->m1, (void*)s->m2, (void*)s->loss, (void*)s->yall})
+        if (q) (void)hipFree(q);
+    delete s;
+}
+
+int kwgpu_surrogate_features(kwgpu_ctx* ctx, const kwgpu_option* opts, size_t n, float* feat) {
+    if (!opts || !feat) return kwgpu::ctx_fail(ctx, KWGPU_ERR_INVALID, "null pointer");
+    for (size_t i = 0; i < n; ++i) (void)option_features(opts[i], feat + i * F);
+    return KWGPU_OK;
+}
+
+int kwgpu_surrogate_get_params(kwgpu_ctx* ctx, const kwgpu_surrogate* s, int32_t member,
+                               float* host) {
+    if (!s || member < 0 || member >= s->members || !host)
+        return kwgpu::ctx_fail(ctx, KWGPU_ERR_INVALID, "kwgpu_surrogate_get_params: bad argument");
+    HIPCHK(hipMemcpy(host, s->p[member], kP * sizeof(float), hipMemcpyDeviceToHost));
+    return KWGPU_OK;
+}
+
+int kwgpu_surrogate_set_params(kwgpu_ctx* ctx, kwgpu_surrogate* s, int32_t member,
+                               const float* host) {
+    if (!s || member < 0 || member >= s->members || !host)
+        return kwgpu::ctx_fail(ctx, KWGPU_ERR_INVALID, "kwgpu_surrogate_set_params: bad argument");
+    hipStream_t st = kwgpu::ctx_stream(ctx);
+    HIPCHK(hipMemcpy(s->p[member], host, kP * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(refresh_bf16(s, member, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return KWGPU_OK;
+}
+
+int kwgpu_surrogate_loss_grad(kwgpu_ctx* ctx, kwgpu_surrogate* s, int32_t member,
+                              const float* host_x, const float* host_t, size_t n, float* host_loss,
+                              float* host_grad) {
+    if (!s || member < 0 || member >= s->members || !host_x || !host_t || n == 0 ||
+        n > (1u << 20))
+        return kwgpu::ctx_fail(ctx, KWGPU_ERR_INVALID, "kwgpu_surrogate_loss_grad: bad argument");
+    hipStream_t st = kwgpu::ctx_stream(ctx);
+    HIPCHK(ensure_scratch(s, n));
+    HIPCHK(hipMemcpy(s->x, host_x, n * F * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(s->tg, host_t, n * sizeof(float), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(zero_f, dim3(1), dim3(64), 0, st, s->loss, (size_t)1);
+    HIPCHK(forward(s, member, (int)n, s->y, s->tg, st));
+    HIPCHK(backward(s, member, (int)n, st));
+    HIPCHK(hipStreamSynchronize(st));
+    float l = 0;
+    HIPCHK(hipMemcpy(&l, s->loss, sizeof(float), hipMemcpyDeviceToHost));
+    if (host_loss) *host_loss = l / (float)n;
+    if (host_grad) HIPCHK(hipMemcpy(host_grad, s->grad, kP * sizeof(float), hipMemcpyDeviceToHost));
+    return KWGPU_OK;
+}
+
+int kwgpu_surrogate_train(kwgpu_ctx* ctx, kwgpu_surrogate* s, const kwgpu_option* d_opts,
+                          const double* d_premium, size_t n, const kwgpu_surrogate_train_params* tp,
+                          float* loss_out) {
+    if (!s || !d_opts || !d_premium || !tp || n == 0 || tp->batch < 64 || tp->epochs < 1 ||
+        n > 0xffffffffu)
+        return kwgpu::ctx_fail(ctx, KWGPU_ERR_INVALID, "kwgpu_surrogate_train: bad argument");
+    HIPCHK(hipSetDevice(s->device));
+    hipStream_t st = kwgpu::ctx_stream(ctx);
+    const int B = tp->batch;
+    float *feat = nullptr, *tgt = nullptr;
+    uint32_t* didx = nullptr;
+    HIPCHK(hipMalloc(&feat, n * F * sizeof(float)));
+    HIPCHK(hipMalloc(&tgt, n * sizeof(float)));
+    HIPCHK(hipMalloc(&didx, n * sizeof(uint32_t)));
+    int rc = KWGPU_OK;
+    do {
+        hipLaunchKernelGGL(features_kernel, dim3(nblk(n, 256)), dim3(256), 0, st, d_opts, n,
+                           d_premium, feat, tgt, (uint8_t*)nullptr);
+        if (hipError_t e = ensure_scratch(s, B)) {
+            rc = kwgpu::ctx_fail(ctx, KWGPU_ERR_NOMEM, std::string("scratch: ") + hipGetErrorString(e));
+            break;
+        }
+        std::vector<uint32_t> perm(n);
+        const size_t steps_per_epoch = n / B;  // drop the ragged tail
+        const double total = (double)steps_per_epoch * tp->epochs;
+        for (int mem = 0; mem < s->members && rc == KWGPU_OK; ++mem) {
+            std::mt19937_64 rng(tp->seed + 7919ull * (uint64_t)mem);
+            hipLaunchKernelGGL(zero_f, dim3(nblk(kP, 256)), dim3(256), 0, st, s->m1, (size_t)kP);
+            hipLaunchKernelGGL(zero_f, dim3(nblk(kP, 256)), dim3(256), 0, st, s->m2, (size_t)kP);
+            long step = 0;
+            for (int ep = 0; ep < tp->epochs && rc == KWGPU_OK; ++ep) {
+                std::iota(perm.begin(), perm.end(), 0u);
+                std::shuffle(perm.begin(), perm.end(), rng);
+                if (hipError_t e = hipMemcpyAsync(didx, perm.data(), n * sizeof(uint32_t),
+                                                  hipMemcpyHostToDevice, st)) {
+                    rc = kwgpu::ctx_fail(ctx, KWGPU_ERR_HIP,
+                                         std::string("upload permutation: ") + hipGetErrorString(e));
+                    break;
+                }
+                hipLaunchKernelGGL(zero_f, dim3(1), dim3(64), 0, st, s->loss, (size_t)1);
+                for (size_t b = 0; b < steps_per_epoch; ++b, ++step) {
+                    hipLaunchKernelGGL(gather_kernel, dim3(nblk(B, 256)), dim3(256), 0, st, feat,
+                                       tgt, didx + b * B, B, s->x, s->tg);
+                    hipError_t e = forward(s, mem, B, s->y, s->tg, st);
+                    if (!e) e = backward(s, mem, B, st);
+                    const double frac = step / total;
+                    const float lr = (float)(tp->lr_final + 0.5 * (tp->lr - tp->lr_final) *
+                                                                 (1.0 + std::cos(M_PI * frac)));
+                    const float c1 = 1.0f - std::pow(tp->beta1, (float)(step + 1));
+                    const float c2 = 1.0f - std::pow(tp->beta2, (float)(step + 1));
+                    hipLaunchKernelGGL(adam_kernel, dim3(nblk(kP, 256)), dim3(256), 0, st,
+                                       s->p[mem], s->grad, s->m1, s->m2, kP, lr, tp->beta1,
+                                       tp->beta2, tp->eps, c1, c2);
+                    if (!e) e = refresh_bf16(s, mem, st);
+                    if (e) {
+                        rc = kwgpu::ctx_fail(ctx, KWGPU_ERR_HIP,
+                                             std::string("train step: ") + hipGetErrorString(e));
+                        break;
+                    }
+                }
+                // the permutation is reused next epoch: wait before overwriting it
+                if (hipError_t e = hipStreamSynchronize(st)) {
+                    rc = kwgpu::ctx_fail(ctx, KWGPU_ERR_HIP, hipGetErrorString(e));
+                    break;
+                }
+                float l = 0;
+                (void)hipMemcpy(&l, s->loss, sizeof(float), hipMemcpyDeviceToHost);
+                if (loss_out) loss_out[(size_t)ep * s->members + mem] = l / (float)(steps_per_epoch * B);
+            }
+        }
+    } while (0);
+    (void)hipStreamSynchronize(st);
+    (void)hipFree(feat);
+    (void)hipFree(tgt);
+    (void)hipFree(didx);
+    return rc;
+}
+
+int kwgpu_surrogate_predict(kwgpu_ctx* ctx, kwgpu_surrogate* s, const kwgpu_option* d_opts,
+                            size_t n, double* d_premium, double* d_spread, uint8_t* d_in_box,
+                            void* stream) {
+    if (!s || !d_opts || !d_premium || !d_spread)
+        return kwgpu::ctx_fail(ctx, KWGPU_ERR_INVALID, "kwgpu_surrogate_predict: bad argument");
+    if (n == 0) return KWGPU_OK;
+    hipStream_t st = stream ? (hipStream_t)stream : kwgpu::ctx_stream(ctx);
+    constexpr size_t CH = 65536;  // samples per pass
+    HIPCHK(ensure_scratch(s, std::min(n, CH)));
+    const size_t need = (size_t)s->members * std::min(n, CH);
+    if (s->yall_cap < need) {
+        if (s->yall) HIPCHK(hipFree(s->yall));
+        s->yall = nullptr;
+        HIPCHK(hipMalloc(&s->yall, need * sizeof(float)));
+        s->yall_cap = need;
+    }
+    for (size_t c0 = 0; c0 < n; c0 += CH) {
+        const int cn = (int)std::min(CH, n - c0);
+        hipLaunchKernelGGL(features_kernel, dim3(nblk(cn, 256)), dim3(256), 0, st, d_opts + c0,
+                           (size_t)cn, (const double*)nullptr, s->x, (float*)nullptr,
+                           d_in_box ? d_in_box + c0 : nullptr);
+        for (int m = 0; m < s->members; ++m)
+            HIPCHK(forward(s, m, cn, s->yall + (size_t)m * cn, nullptr, st));
+        hipLaunchKernelGGL(combine_kernel, dim3(nblk(cn, 256)), dim3(256), 0, st, d_opts + c0,
+                           s->yall, s->members, (size_t)cn, (size_t)cn, d_premium + c0,
+                           d_spread + c0);
+        HIPCHK(hipGetLastError());
+    }
+    return KWGPU_OK;
+}
+
+int kwgpu_surrogate_price(kwgpu_ctx* ctx, kwgpu_surrogate* s, const kwgpu_option* d_opts,
+                          size_t n, double spread_max, const kwgpu_fd1d_params* fd,
+                          double* d_prices, int32_t* d_route, size_t* n_fd) {
+    if (!s || !d_opts || !d_prices || !d_route || !fd)
+        return kwgpu::ctx_fail(ctx, KWGPU_ERR_INVALID, "kwgpu_surrogate_price: bad argument");
+    if (n_fd) *n_fd = 0;
+    if (n == 0) return KWGPU_OK;
+    if (n > 0x7fffffffu)
+        return kwgpu::ctx_fail(ctx, KWGPU_ERR_INVALID, "kwgpu_surrogate_price: too many options");
+    hipStream_t st = kwgpu::ctx_stream(ctx);
+    double *prem = nullptr, *spr = nullptr, *sub_p = nullptr;
+    uint8_t* inb = nullptr;
+    uint32_t *idx = nullptr;
+    int* cnt = nullptr;
+    int32_t* sub_s = nullptr;
+    kwgpu_option* sub_o = nullptr;
+    int rc = KWGPU_OK;
+    auto cleanup = [&]() {
+        (void)hipStreamSynchronize(st);
+        for (void* q : {(void*)prem, (void*)spr, (void*)inb, (void*)idx, (void*)cnt,
+                        (void*)sub_p, (void*)sub_s, (void*)sub_o})
+            if (q) (void)hipFree(q);
+    };
+    if (hipMalloc(&prem, n * sizeof(double)) || hipMalloc(&spr, n * sizeof(double)) ||
+        hipMalloc(&inb, n) || hipMalloc(&idx, 2 * n * sizeof(uint32_t)) ||
+        hipMalloc(&cnt, 2 * sizeof(int)) || hipMalloc(&sub_p, n * sizeof(double)) ||
+        hipMalloc(&sub_s, n * sizeof(int32_t)) || hipMalloc(&sub_o, n * sizeof(kwgpu_option))) {
+        cleanup();
+        return kwgpu::ctx_fail(ctx, KWGPU_ERR_NOMEM, "kwgpu_surrogate_price: allocation");
+    }
+    rc = kwgpu_surrogate_predict(ctx, s, d_opts, n, prem, spr, inb, st);
+    if (rc) {
+        cleanup();
+        return rc;
+    }
+    hipLaunchKernelGGL(zero_f, dim3(1), dim3(64), 0, st, (float*)cnt, (size_t)2);
+    // exp(scale * sigma sqrt(T)) must stay below the FP32 range (~3.4e38 = e^88.7)
+    const double fp64_above = 80.0 / fd->scale;
+    hipLaunchKernelGGL(gate_kernel, dim3(nblk(n, 256)), dim3(256), 0, st, d_opts, n, prem, spr, inb,
+                       spread_max, fp64_above, d_prices, d_route, cnt, idx, idx + n);
+    int hc[2] = {0, 0};
+    if (hipMemcpyAsync(hc, cnt, sizeof(hc), hipMemcpyDeviceToHost, st) ||
+        hipStreamSynchronize(st)) {
+        cleanup();
+        return kwgpu::ctx_fail(ctx, KWGPU_ERR_HIP, "kwgpu_surrogate_price: gate");
+    }
+    for (int r = 0; r < 2 && rc == KWGPU_OK; ++r) {
+        const int m = hc[r];
+        if (m == 0) continue;
+        const uint32_t* ix = idx + (size_t)r * n;
+        hipLaunchKernelGGL(gather_opts, dim3(nblk(m, 256)), dim3(256), 0, st, d_opts, ix, m, sub_o);
+        kwgpu_fd1d_params p = *fd;
+        p.precision = r == 0 ? KWGPU_FP32_MIXED : KWGPU_FP64;
+        p.x_dims = nullptr;  // per-option sizes would need the same gather
+        p.t_dims = nullptr;
+        rc = kwgpu_fd1d_price_device(ctx, sub_o, (size_t)m, &p, sub_p, sub_s, st);
+        if (rc) break;
+        hipLaunchKernelGGL(scatter_prices, dim3(nblk(m, 256)), dim3(256), 0, st, sub_p, sub_s, ix, m,
+                           d_prices);
+        if (hipStreamSynchronize(st)) rc = kwgpu::ctx_fail(ctx, KWGPU_ERR_HIP, "FD fallback");
+    }
+    if (n_fd) *n_fd = (size_t)hc[0] + (size_t)hc[1];
+    cleanup();
+    return rc;
+}
+
+int kwgpu_surrogate_save(kwgpu_ctx* ctx, const kwgpu_surrogate* s, const char* path) {
+    if (!s || !path) return kwgpu::ctx_fail(ctx, KWGPU_ERR_INVALID, "kwgpu_surrogate_save: bad argument");
+    FILE* f = std::fopen(path, "wb");
+    if (!f) return kwgpu::ctx_fail(ctx, KWGPU_ERR_INVALID, std::string("cannot write ") + path);
+    const uint32_t hdr[4] = {0x4b575355u /* "KWSU" */, 1u, (uint32_t)s->members, (uint32_t)kP};
+    std::fwrite(hdr, sizeof(hdr), 1, f);
+    std::vector<float> w(kP);
+    for (int m = 0; m < s->members; ++m) {
+        if (hipMemcpy(w.data(), s->p[m], kP * sizeof(float), hipMemcpyDeviceToHost)) {
+            std::fclose(f);
+            return kwgpu::ctx_fail(ctx, KWGPU_ERR_HIP, "kwgpu_surrogate_save: download");
+        }
+        std::fwrite(w.data(), sizeof(float), kP, f);
+    }
+    std::fclose(f);
+    return KWGPU_OK;
+}
+
+int kwgpu_surrogate_load(kwgpu_ctx* ctx, const char* path, kwgpu_surrogate** out) {
+    if (!path || !out) return kwgpu::ctx_fail(ctx, KWGPU_ERR_INVALID, "kwgpu_surrogate_load: bad argument");
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return kwgpu::ctx_fail(ctx, KWGPU_ERR_INVALID, std::string("cannot read ") + path);
+    uint32_t hdr[4] = {};
+    if (std::fread(hdr, sizeof(hdr), 1, f) != 1 || hdr[0] != 0x4b575355u || hdr[1] != 1u ||
+        hdr[3] != (uint32_t)kP || hdr[2] < 1 || hdr[2] > 16) {
+        std::fclose(f);
+        return kwgpu::ctx_fail(ctx, KWGPU_ERR_INVALID, "kwgpu_surrogate_load: not a surrogate file");
+    }
+    int rc = kwgpu_surrogate_create(ctx, (int32_t)hdr[2], 0, out);
+    std::vector<float> w(kP);
+    for (uint32_t m = 0; rc == KWGPU_OK && m < hdr[2]; ++m) {
+        if (std::fread(w.data(), sizeof(float), kP, f) != (size_t)kP)
+            rc = kwgpu::ctx_fail(ctx, KWGPU_ERR_INVALID, "kwgpu_surrogate_load: truncated file");
+        else
+            rc = kwgpu_surrogate_set_params(ctx, *out, (int32_t)m, w.data());
+    }
+    std::fclose(f);
+    if (rc != KWGPU_OK && *out) {
+        kwgpu_surrogate_destroy(*out);
+        *out = nullptr;
+    }
+    return rc;
+}
+
+}  // extern "C"

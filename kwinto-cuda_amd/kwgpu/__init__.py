@@ -26,7 +26,21 @@ EXPORTED_SYMBOLS = (
     "kwgpu_fd1d_params_default", "kwgpu_create", "kwgpu_destroy", "kwgpu_last_error",
     "kwgpu_fd1d_price", "kwgpu_fd1d_price_device", "kwgpu_bs_price", "kwgpu_bs_price_device",
     "kwgpu_fd1d_bs_price", "kwgpu_synchronize", "kwgpu_stream",
+    "kwgpu_surrogate_train_params_default", "kwgpu_surrogate_create", "kwgpu_surrogate_destroy",
+    "kwgpu_surrogate_train", "kwgpu_surrogate_predict", "kwgpu_surrogate_price",
+    "kwgpu_surrogate_get_params", "kwgpu_surrogate_set_params", "kwgpu_surrogate_loss_grad",
+    "kwgpu_surrogate_features", "kwgpu_surrogate_save", "kwgpu_surrogate_load",
+    "kwgpu_fd1d_autotune", "kwgpu_tune_save", "kwgpu_tune_load",
 )
+SURROGATE_PARAMS = 199425
+
+
+class SurrogateTrainParams(ctypes.Structure):
+    _fields_ = [
+        ("epochs", ctypes.c_int32), ("batch", ctypes.c_int32), ("lr", ctypes.c_float),
+        ("lr_final", ctypes.c_float), ("beta1", ctypes.c_float), ("beta2", ctypes.c_float),
+        ("eps", ctypes.c_float), ("seed", ctypes.c_uint64),
+    ]
 
 
 class Fd1dParams(ctypes.Structure):
@@ -46,6 +60,12 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"kwgpu: {LIB_PATH} not built; run __graft_entry__.build()")
+        # One HIP runtime serves the whole process and the first one loaded wins: load
+        # torch's (it works only with its own) before this library when torch is present.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         P, SZ = ctypes.c_void_p, ctypes.c_size_t
         L.kwgpu_fd1d_params_default.argtypes = [ctypes.POINTER(Fd1dParams)]
@@ -63,6 +83,26 @@ def lib():
         L.kwgpu_synchronize.argtypes = [P]
         L.kwgpu_stream.argtypes = [P]
         L.kwgpu_stream.restype = P
+        L.kwgpu_surrogate_train_params_default.argtypes = [ctypes.POINTER(SurrogateTrainParams)]
+        L.kwgpu_surrogate_train_params_default.restype = None
+        L.kwgpu_surrogate_create.argtypes = [P, ctypes.c_int32, ctypes.c_uint64, ctypes.POINTER(P)]
+        L.kwgpu_surrogate_destroy.argtypes = [P]
+        L.kwgpu_surrogate_destroy.restype = None
+        L.kwgpu_surrogate_train.argtypes = [P, P, P, P, SZ, ctypes.POINTER(SurrogateTrainParams), P]
+        L.kwgpu_surrogate_predict.argtypes = [P, P, P, SZ, P, P, P, P]
+        L.kwgpu_surrogate_price.argtypes = [P, P, P, SZ, ctypes.c_double,
+                                            ctypes.POINTER(Fd1dParams), P, P,
+                                            ctypes.POINTER(ctypes.c_size_t)]
+        L.kwgpu_surrogate_get_params.argtypes = [P, P, ctypes.c_int32, P]
+        L.kwgpu_surrogate_set_params.argtypes = [P, P, ctypes.c_int32, P]
+        L.kwgpu_surrogate_loss_grad.argtypes = [P, P, ctypes.c_int32, P, P, SZ, P, P]
+        L.kwgpu_surrogate_features.argtypes = [P, P, SZ, P]
+        L.kwgpu_surrogate_save.argtypes = [P, P, ctypes.c_char_p]
+        L.kwgpu_surrogate_load.argtypes = [P, ctypes.c_char_p, ctypes.POINTER(P)]
+        L.kwgpu_fd1d_autotune.argtypes = [P, P, SZ, ctypes.POINTER(Fd1dParams), P, P,
+                                          ctypes.c_char_p, SZ]
+        L.kwgpu_tune_save.argtypes = [P, ctypes.c_char_p]
+        L.kwgpu_tune_load.argtypes = [P, ctypes.c_char_p]
         _lib = L
     return _lib
 
@@ -176,7 +216,125 @@ class Context:
         if rc != OK:
             raise KwgpuError(rc, self.last_error)
 
+    def autotune(self, d_opts: int, n: int, d_prices: int, d_status: int | None,
+                 p: Fd1dParams) -> dict:
+        """Times the kernel variants per grid-size bucket and keeps the fastest."""
+        import json
+        buf = ctypes.create_string_buffer(1 << 16)
+        rc = lib().kwgpu_fd1d_autotune(self.h, d_opts, n, ctypes.byref(p), d_prices, d_status,
+                                       buf, len(buf))
+        if rc != OK:
+            raise KwgpuError(rc, self.last_error)
+        return json.loads(buf.value.decode())
+
+    def tune_save(self, path: str):
+        rc = lib().kwgpu_tune_save(self.h, path.encode())
+        if rc != OK:
+            raise KwgpuError(rc, self.last_error)
+
+    def tune_load(self, path: str):
+        rc = lib().kwgpu_tune_load(self.h, path.encode())
+        if rc != OK:
+            raise KwgpuError(rc, self.last_error)
+
     def bs_price_device(self, d_opts: int, n: int, d_prices: int, stream: int | None = None):
         rc = lib().kwgpu_bs_price_device(self.h, d_opts, n, d_prices, stream)
         if rc != OK:
             raise KwgpuError(rc, self.last_error)
+
+
+class Surrogate:
+    """An ensemble of premium networks on a Context (see kwgpu_surrogate_* in kwgpu.h)."""
+
+    def __init__(self, ctx: Context, members: int = 4, seed: int = 1, _handle=None):
+        self.ctx = ctx
+        if _handle is not None:
+            self.h = _handle
+        else:
+            h = ctypes.c_void_p()
+            rc = lib().kwgpu_surrogate_create(ctx.h, members, seed, ctypes.byref(h))
+            if rc != OK:
+                raise KwgpuError(rc, ctx.last_error)
+            self.h = h
+        self.members = members
+
+    @classmethod
+    def load(cls, ctx: Context, path: str) -> "Surrogate":
+        h = ctypes.c_void_p()
+        rc = lib().kwgpu_surrogate_load(ctx.h, path.encode(), ctypes.byref(h))
+        if rc != OK:
+            raise KwgpuError(rc, ctx.last_error)
+        with open(path, "rb") as f:
+            members = int(np.frombuffer(f.read(16), np.uint32)[2])
+        return cls(ctx, members, _handle=h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().kwgpu_surrogate_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != OK:
+            raise KwgpuError(rc, self.ctx.last_error)
+
+    def save(self, path: str):
+        self._check(lib().kwgpu_surrogate_save(self.ctx.h, self.h, path.encode()))
+
+    def train(self, d_opts: int, d_premium: int, n: int, epochs=30, batch=8192, lr=2e-3,
+              lr_final=2e-5, seed=7) -> np.ndarray:
+        tp = SurrogateTrainParams()
+        lib().kwgpu_surrogate_train_params_default(ctypes.byref(tp))
+        tp.epochs, tp.batch, tp.lr, tp.lr_final, tp.seed = epochs, batch, lr, lr_final, seed
+        loss = np.zeros((epochs, self.members), np.float32)
+        self._check(lib().kwgpu_surrogate_train(self.ctx.h, self.h, d_opts, d_premium, n,
+                                                ctypes.byref(tp), loss.ctypes.data))
+        return loss
+
+    def predict(self, d_opts: int, n: int, d_premium: int, d_spread: int, d_in_box: int,
+                stream: int | None = None):
+        self._check(lib().kwgpu_surrogate_predict(self.ctx.h, self.h, d_opts, n, d_premium,
+                                                  d_spread, d_in_box, stream))
+
+    def price(self, d_opts: int, n: int, spread_max: float, fd: Fd1dParams, d_prices: int,
+              d_route: int) -> int:
+        nfd = ctypes.c_size_t(0)
+        self._check(lib().kwgpu_surrogate_price(self.ctx.h, self.h, d_opts, n, spread_max,
+                                                ctypes.byref(fd), d_prices, d_route,
+                                                ctypes.byref(nfd)))
+        return nfd.value
+
+    def get_params(self, member: int) -> np.ndarray:
+        w = np.zeros(SURROGATE_PARAMS, np.float32)
+        self._check(lib().kwgpu_surrogate_get_params(self.ctx.h, self.h, member, w.ctypes.data))
+        return w
+
+    def set_params(self, member: int, w):
+        w = np.ascontiguousarray(w, np.float32)
+        assert w.size == SURROGATE_PARAMS
+        self._check(lib().kwgpu_surrogate_set_params(self.ctx.h, self.h, member, w.ctypes.data))
+
+    def loss_grad(self, member: int, x, t):
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        loss = ctypes.c_float()
+        g = np.zeros(SURROGATE_PARAMS, np.float32)
+        self._check(lib().kwgpu_surrogate_loss_grad(self.ctx.h, self.h, member, x.ctypes.data,
+                                                    t.ctypes.data, len(t), ctypes.byref(loss),
+                                                    g.ctypes.data))
+        return float(loss.value), g
+
+
+def surrogate_features(opts) -> np.ndarray:
+    """Normalised network inputs (n x 6) of host options, as the kernels compute them."""
+    a = as_options(opts)
+    out = np.zeros((len(a), 6), np.float32)
+    rc = lib().kwgpu_surrogate_features(None, a.ctypes.data, len(a), out.ctypes.data)
+    if rc != OK:
+        raise KwgpuError(rc, "kwgpu_surrogate_features failed")
+    return out

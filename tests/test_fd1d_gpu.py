@@ -288,3 +288,74 @@ def test_mixed_full_size_per_class(ctx):
             _say(f"  w={w:+d} T in [{lo},{hi}): n={m.sum()} max={d[m].max():.3e} "
                  f"rms={np.sqrt(np.mean(d[m] ** 2)):.3e}")
     assert d.max() <= MIXED_TOL
+
+
+# ---- uneven batches: per-bucket kernels, persistent grids, hipGraph replay -------------
+
+def _uneven_device(o, xd, td):
+    import torch
+    d_o = torch.from_numpy(o.view(np.uint8).copy()).cuda()
+    d_x = torch.from_numpy(xd.astype(np.int32)).cuda()
+    d_t = torch.from_numpy(td.astype(np.int32)).cuda()
+    return d_o, d_x, d_t
+
+
+@pytest.mark.parametrize("precision", [kwgpu.FP64, kwgpu.FP32_MIXED], ids=["fp64", "mixed"])
+def test_uneven_device_graph_replay(ctx, precision, monkeypatch):
+    """Device path with per-option grids: matches the oracle (one reference call per grid
+    size), replays its captured graph bitwise, and equals direct launches."""
+    import torch
+    _say("uneven device", precision)
+    o, xd, td = W.uneven(3000, 77)
+    td = np.minimum(td, 64)  # keep the oracle check quick; sizes still span all buckets
+    d_o, d_x, d_t = _uneven_device(o, xd, td)
+    n = len(o)
+    d_p = torch.empty(n, dtype=torch.float64, device="cuda")
+    d_s = torch.empty(n, dtype=torch.int32, device="cuda")
+    prm = kwgpu.params(precision=precision, x_dims=d_x.data_ptr(), t_dims=d_t.data_ptr())
+    runs = []
+    for _ in range(3):
+        d_p.fill_(-1.0)
+        ctx.fd1d_price_device(d_o.data_ptr(), n, d_p.data_ptr(), d_s.data_ptr(), prm)
+        ctx.synchronize()
+        runs.append(d_p.cpu().numpy().copy())
+    assert np.all(d_s.cpu().numpy() == 0)
+    assert np.array_equal(runs[0], runs[1]) and np.array_equal(runs[1], runs[2])
+    sub = np.arange(0, n, 7)
+    ref, st = oracle.fd1d_price_uneven(o[sub], xd[sub], td[sub])
+    tol = EPS if precision == kwgpu.FP64 else 1e-4
+    assert np.all(st == 0)
+    assert np.max(np.abs(runs[0][sub] - ref)) <= tol
+    monkeypatch.setenv("KWGPU_NO_GRAPH", "1")
+    direct = kwgpu.Context(0)
+    direct.fd1d_price_device(d_o.data_ptr(), n, d_p.data_ptr(), d_s.data_ptr(), prm)
+    direct.synchronize()
+    assert np.array_equal(d_p.cpu().numpy(), runs[0])
+    direct.close()
+
+
+def test_autotune_keeps_results(ctx):
+    import torch
+    _say("autotune")
+    o, xd, td = W.uneven(6000, 78)
+    td = np.minimum(td, 48)
+    d_o, d_x, d_t = _uneven_device(o, xd, td)
+    n = len(o)
+    d_p = torch.empty(n, dtype=torch.float64, device="cuda")
+    d_s = torch.empty(n, dtype=torch.int32, device="cuda")
+    c2 = kwgpu.Context(0)
+    prm = kwgpu.params(x_dims=d_x.data_ptr(), t_dims=d_t.data_ptr())
+    c2.fd1d_price_device(d_o.data_ptr(), n, d_p.data_ptr(), d_s.data_ptr(), prm)
+    c2.synchronize()
+    before = d_p.cpu().numpy().copy()
+    rep = c2.autotune(d_o.data_ptr(), n, d_p.data_ptr(), d_s.data_ptr(), prm)
+    _say("  ", rep)
+    assert {b["bucket"] for b in rep["buckets"]} == {256, 512, 1024, 2048}
+    for b in rep["buckets"]:
+        assert all(c["ms"] > 0 for c in b["candidates"])
+    c2.fd1d_price_device(d_o.data_ptr(), n, d_p.data_ptr(), d_s.data_ptr(), prm)
+    c2.synchronize()
+    after = d_p.cpu().numpy()
+    assert np.all(d_s.cpu().numpy() == 0)
+    assert np.max(np.abs(after - before)) <= EPS
+    c2.close()

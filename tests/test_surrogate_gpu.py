@@ -1,0 +1,232 @@
+"""The premium network's BF16-MFMA forward/backward kernels, its training, and the
+surrogate-first serving path, through the C-ABI.
+
+Gradient oracle: an explicit float64 restatement of the network's forward and
+backward pass with BF16 rounding (round-to-nearest-even) applied at exactly the points
+the kernels round (activations entering the 256x256 GEMMs, the BF16 weight copies, the
+layer deltas), so the comparison can be tight; plus torch's FP32 autograd without any
+BF16 rounding as an independent, looser check.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import kwgpu
+from kwgpu import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+H, F = 256, 6
+kW1, kB1 = 0, 256 * 6
+kW2 = kB1 + H
+kB2 = kW2 + H * H
+kW3 = kB2 + H
+kB3 = kW3 + H * H
+kW4 = kB3 + H
+kB4 = kW4 + H * H
+kW5 = kB4 + H
+kB5 = kW5 + H
+NP = kB5 + 1
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def bf16(a):
+    """Round to bfloat16 (round to nearest even) and back, as v_cvt to bf16 does."""
+    u = np.ascontiguousarray(a, np.float32).view(np.uint32).astype(np.uint64)
+    u = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def unpack(w):
+    w = w.astype(np.float64)
+    return dict(W1=w[kW1:kB1].reshape(H, F), b1=w[kB1:kW2], W2=w[kW2:kB2].reshape(H, H),
+                b2=w[kB2:kW3], W3=w[kW3:kB3].reshape(H, H), b3=w[kB3:kW4],
+                W4=w[kW4:kB4].reshape(H, H), b4=w[kB4:kW5], W5=w[kW5:kB5], b5=w[kB5])
+
+
+def sig(z):
+    return 1.0 / (1.0 + np.exp(-z))
+
+
+def silu(z):
+    return z * sig(z)
+
+
+def dsilu(z):
+    s = sig(z)
+    return s * (1 + z * (1 - s))
+
+
+def oracle_loss_grad(w, x, t):
+    p = unpack(w)
+    n = len(t)
+    x = x.astype(np.float64)
+    z = [x @ p["W1"].T + p["b1"]]
+    h = [bf16(silu(z[0]))]
+    for l in (2, 3, 4):
+        z.append(h[-1] @ bf16(p[f"W{l}"]).T + p[f"b{l}"])
+        h.append(bf16(silu(z[-1])))
+    y = silu(z[3]) @ p["W5"] + p["b5"]
+    loss = np.mean((y - t) ** 2)
+    dy = 2 * (y - t) / n
+    g = {"W5": silu(z[3]).T @ dy, "b5": np.sum(dy)}
+    dz = bf16(np.outer(dy, p["W5"]) * dsilu(z[3]))
+    for l in (4, 3, 2):
+        g[f"W{l}"] = dz.T @ h[l - 2]
+        g[f"b{l}"] = dz.sum(0)
+        dz = bf16((dz @ bf16(p[f"W{l}"])) * dsilu(z[l - 2]))
+    g["W1"] = dz.T @ x
+    g["b1"] = dz.sum(0)
+    flat = np.concatenate([g["W1"].ravel(), g["b1"], g["W2"].ravel(), g["b2"], g["W3"].ravel(),
+                           g["b3"], g["W4"].ravel(), g["b4"], g["W5"], [g["b5"]]])
+    return loss, flat
+
+
+SLICES = {"W1": (kW1, kB1), "b1": (kB1, kW2), "W2": (kW2, kB2), "b2": (kB2, kW3),
+          "W3": (kW3, kB3), "b3": (kB3, kW4), "W4": (kW4, kB4), "b4": (kB4, kW5),
+          "W5": (kW5, kB5), "b5": (kB5, NP)}
+
+
+@pytest.fixture(scope="module")
+def sur(ctx):
+    s = kwgpu.Surrogate(ctx, members=2, seed=11)
+    yield s
+    s.close()
+
+
+@pytest.mark.parametrize("n", [64, 333, 4096])
+def test_loss_and_gradient_match_oracle(sur, n):
+    _say("loss/grad n =", n)
+    rng = np.random.default_rng(n)
+    w = sur.get_params(0)
+    # make the output layer and biases non-trivial so every path carries signal
+    w[kW5:kB5] = rng.uniform(-0.3, 0.3, H).astype(np.float32)
+    w[kB2:kW3] = rng.uniform(-0.1, 0.1, H).astype(np.float32)
+    sur.set_params(0, w)
+    x = rng.uniform(-1, 1, (n, F)).astype(np.float32)
+    x[:, 5] = np.sign(x[:, 5])
+    t = rng.normal(0, 0.5, n).astype(np.float32)
+    loss, g = sur.loss_grad(0, x, t)
+    rl, rg = oracle_loss_grad(w, x, t)
+    assert abs(loss - rl) <= 1e-4 * abs(rl)
+    for name, (a, b) in SLICES.items():
+        ga, gr = g[a:b].astype(np.float64), rg[a:b]
+        err = np.linalg.norm(ga - gr) / max(np.linalg.norm(gr), 1e-30)
+        _say(f"  {name}: rel err {err:.2e}")
+        assert err < 5e-3, name
+
+
+def test_gradient_close_to_fp32_autograd(sur):
+    torch = pytest.importorskip("torch")
+    rng = np.random.default_rng(5)
+    n = 1024
+    w = sur.get_params(1)
+    x = rng.uniform(-1, 1, (n, F)).astype(np.float32)
+    t = rng.normal(0, 0.5, n).astype(np.float32)
+    _, g = sur.loss_grad(1, x, t)
+    tw = torch.tensor(w, dtype=torch.float32, requires_grad=True)
+    p = {k: tw[a:b] for k, (a, b) in SLICES.items()}
+    act = torch.nn.functional.silu
+    h = act(torch.tensor(x) @ p["W1"].view(H, F).T + p["b1"])
+    for l in (2, 3):
+        h = act(h @ p[f"W{l}"].view(H, H).T + p[f"b{l}"])
+    z4 = h @ p["W4"].view(H, H).T + p["b4"]
+    y = act(z4) @ p["W5"] + p["b5"]
+    loss = torch.mean((y - torch.tensor(t)) ** 2)
+    loss.backward()
+    ref = tw.grad.numpy().astype(np.float64)
+    cos = np.dot(ref, g) / (np.linalg.norm(ref) * np.linalg.norm(g))
+    assert cos > 0.995, cos
+
+
+def _fd_premium(ctx, opts, x_dim, t_dim, precision=kwgpu.FP64):
+    """FD American price minus BS European price, both on the GPU."""
+    import torch
+    n = len(opts)
+    d_o = torch.from_numpy(opts.view(np.uint8).copy()).cuda()
+    d_p = torch.empty(n, dtype=torch.float64, device="cuda")
+    d_s = torch.empty(n, dtype=torch.int32, device="cuda")
+    ctx.fd1d_price_device(d_o.data_ptr(), n, d_p.data_ptr(), d_s.data_ptr(),
+                          kwgpu.params(x_dim, t_dim, precision=precision))
+    e = opts.copy()
+    e["e"] = 0
+    d_e = torch.from_numpy(e.view(np.uint8).copy()).cuda()
+    d_b = torch.empty(n, dtype=torch.float64, device="cuda")
+    ctx.bs_price_device(d_e.data_ptr(), n, d_b.data_ptr())
+    ctx.synchronize()
+    assert np.all(d_s.cpu().numpy() == 0)
+    return d_o, d_p - d_b
+
+
+def test_training_learns_the_premium(ctx):
+    """A short training run on FD labels (256 x 128 grid) lowers the loss and predicts
+    held-out premiums far better than their mean."""
+    import torch
+    _say("training")
+    o = W.surrogate_train(40000, 31)
+    d_o, prem = _fd_premium(ctx, o, 256, 128)
+    n_tr = 32768
+    s = kwgpu.Surrogate(ctx, members=2, seed=3)
+    loss = s.train(d_o.data_ptr(), prem.data_ptr(), n_tr, epochs=12, batch=2048, lr=2e-3)
+    _say("  loss per epoch", loss[:, 0])
+    assert np.all(np.isfinite(loss))
+    assert loss[-1].max() < loss[0].min() / 4
+    n_te = len(o) - n_tr
+    d_te = d_o[n_tr * 56:]
+    d_pm = torch.empty(n_te, dtype=torch.float64, device="cuda")
+    d_sp = torch.empty(n_te, dtype=torch.float64, device="cuda")
+    d_ib = torch.empty(n_te, dtype=torch.uint8, device="cuda")
+    s.predict(d_te.data_ptr(), n_te, d_pm.data_ptr(), d_sp.data_ptr(), d_ib.data_ptr())
+    ctx.synchronize()
+    truth = prem[n_tr:].cpu().numpy()
+    pred = d_pm.cpu().numpy()
+    assert np.all(d_ib.cpu().numpy() == 1)
+    rms = np.sqrt(np.mean((pred - truth) ** 2))
+    base = np.sqrt(np.mean((truth - truth.mean()) ** 2))
+    _say(f"  held-out rms {rms:.3e} vs spread of the labels {base:.3e}")
+    assert rms < base / 10
+    assert np.all(d_sp.cpu().numpy() >= 0)
+    s.close()
+
+
+def test_serving_routes_and_fallback(ctx, tmp_path):
+    import torch
+    _say("serving")
+    o = W.serving(4096, 41)
+    n = len(o)
+    d_o = torch.from_numpy(o.view(np.uint8).copy()).cuda()
+    s = kwgpu.Surrogate(ctx, members=2, seed=4)
+    fd = kwgpu.params(256, 64)
+    d_p = torch.empty(n, dtype=torch.float64, device="cuda")
+    d_r = torch.empty(n, dtype=torch.int32, device="cuda")
+    # nothing passes a negative spread bound: every option is priced by FD
+    nfd = s.price(d_o.data_ptr(), n, -1.0, fd, d_p.data_ptr(), d_r.data_ptr())
+    assert nfd == n
+    route = d_r.cpu().numpy()
+    assert np.all(route >= 1)
+    p_all_fd = d_p.cpu().numpy()
+    ref, _, rc, _ = ctx.fd1d_price(o, 256, 64, precision=kwgpu.FP64)
+    assert rc == 0
+    assert np.max(np.abs(p_all_fd - ref)) <= 1e-4
+    # everything in the box passes an infinite bound; out-of-box options still go to FD
+    feats = kwgpu.surrogate_features(o)
+    in_box = np.all(np.abs(feats) <= 1 + 1e-9, axis=1)
+    nfd = s.price(d_o.data_ptr(), n, 1e300, fd, d_p.data_ptr(), d_r.data_ptr())
+    route = d_r.cpu().numpy()
+    assert nfd == int((~in_box).sum())
+    assert np.array_equal(route == 0, in_box)
+    p = d_p.cpu().numpy()
+    assert np.max(np.abs(p[~in_box] - ref[~in_box])) <= 1e-4
+    # save / load gives the same predictions
+    path = str(tmp_path / "sur.bin")
+    s.save(path)
+    s2 = kwgpu.Surrogate.load(ctx, path)
+    d_p2 = torch.empty(n, dtype=torch.float64, device="cuda")
+    s2.price(d_o.data_ptr(), n, 1e300, fd, d_p2.data_ptr(), d_r.data_ptr())
+    assert np.array_equal(d_p2.cpu().numpy()[in_box], p[in_box])
+    s.close()
+    s2.close()

@@ -280,7 +280,8 @@ def main():
             "unit": "TFLOP/s",
             "frac": achieved / FP64_PEAK_TFLOPS,
             "traffic": traffic,
-            "kernel": "fd1d_wave_f64<16,64>",
+            "kernel": ("fd1d_lane_f64" if args.layout == "lane" else
+                       "fd1d_wave<M=16,L=64,W=1,PERSIST=false,MIXED=false>"),
             "kernel_ms": avg_kernel_s * 1e3,
             "algorithmic_flops_per_launch": flops,
         },

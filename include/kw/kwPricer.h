@@ -121,7 +121,21 @@ public:
             return m_gpu.error();
         return "";
     }
-    /// Per-option status of the last price() call (KWGPU_STATUS_*).
+    /// Price and bump Greeks (test/portfolio.py:35-56) of every option, all nine bumped
+    /// prices per option solved in one batch; spot bumps share the option's PDE.
+    Error greeks(const std::vector<Option>& assets, std::vector<kwgpu_greeks>& out,
+                 f64 bump = 0.01) {
+        const auto n = assets.size();
+        out.resize(n);
+        if (n == 0) return "";
+        if (!m_gpu.get()) return "Fd1d_Pricer::greeks: not initialised";
+        m_status.resize(n);
+        if (kwgpu_fd1d_greeks(m_gpu.get(), reinterpret_cast<const kwgpu_option*>(assets.data()), n,
+                              &m_params, bump, out.data(), m_status.data()) != KWGPU_OK)
+            return m_gpu.error();
+        return "";
+    }
+    /// Per-option status of the last price() or greeks() call (KWGPU_STATUS_*).
     const std::vector<int32_t>& status() const { return m_status; }
 };
 

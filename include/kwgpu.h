@@ -142,6 +142,29 @@ int kwgpu_fd1d_bs_price(kwgpu_ctx* ctx, const kwgpu_option* opts, size_t n,
                         const kwgpu_fd1d_params* params, double* prices, int32_t* status);
 
 /*
+ * Price and Greeks by central bumps of relative size `bump`, as defined by the
+ * reference's test/portfolio.py:35-56 (delta, gamma, rho, theta, vega; bump 0.01 there):
+ *   delta = (V(s(1+b)) - V(s(1-b))) / (2 b s)
+ *   gamma = (V(s(1+b)) - 2 V + V(s(1-b))) / (b s)^2
+ *   vega  = (V(z(1+b)) - V(z(1-b))) / (2 b z) / 100
+ *   theta = (V(t(1+b)) - V(t(1-b))) / (2 b t) / 360     (derivative in expiry)
+ *   rho   = (V(r(1+b)) - V(r(1-b))) / (2 b r) / 100
+ * The spot bumps share each option's PDE (the PDE does not depend on s), so delta and
+ * gamma come from the solved grid at no extra solve; vega, theta and rho cost two PDEs
+ * each. All nine prices of every option are solved in one batch. A zero s, z, t or r
+ * gives NaN for the Greek divided by it (the script would raise). `status` (may be
+ * NULL) receives the first non-OK status of an option's nine prices; the return value
+ * follows kwgpu_fd1d_price, with the failing bumped option in the message.
+ */
+typedef struct kwgpu_greeks {
+    double price, delta, gamma, vega, theta, rho;
+} kwgpu_greeks;
+
+int kwgpu_fd1d_greeks(kwgpu_ctx* ctx, const kwgpu_option* opts, size_t n,
+                      const kwgpu_fd1d_params* params, double bump, kwgpu_greeks* out,
+                      int32_t* status);
+
+/*
  * Autotuner (no counterpart in the reference). For every grid-size bucket of this
  * batch (x-points rounded up to a power of two, 64 .. 2048), times on a sample of the
  * bucket's options each wave-layout variant that holds it (rows per lane x lanes per

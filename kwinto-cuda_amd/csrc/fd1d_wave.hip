@@ -57,17 +57,21 @@ constexpr int kWaveShr1 = 0x138;
 constexpr int kWaveShl1 = 0x130;
 
 // DPP move of a float or a double (two 32-bit v_mov_b32_dpp). Lanes whose source is
-// outside the row/wave read 0 (bound_ctrl); lanes disabled by ROWMASK read 0.
+// outside the row/wave read 0 (bound_ctrl); lanes in rows disabled by ROWMASK keep
+// `old`, so a caller that multiplies the result by 0 there can pass any finite value
+// already in a register and save the zeroing moves.
 template <int CTRL, int ROWMASK = 0xF, class T>
-__device__ __forceinline__ T dpp(T v) {
+__device__ __forceinline__ T dpp(T v, T old = T(0)) {
     if constexpr (sizeof(T) == 8) {
         int2 a = __builtin_bit_cast(int2, v);
-        a.x = __builtin_amdgcn_update_dpp(0, a.x, CTRL, ROWMASK, 0xF, true);
-        a.y = __builtin_amdgcn_update_dpp(0, a.y, CTRL, ROWMASK, 0xF, true);
+        const int2 o = __builtin_bit_cast(int2, old);
+        a.x = __builtin_amdgcn_update_dpp(o.x, a.x, CTRL, ROWMASK, 0xF, true);
+        a.y = __builtin_amdgcn_update_dpp(o.y, a.y, CTRL, ROWMASK, 0xF, true);
         return __builtin_bit_cast(T, a);
     } else {
         return __builtin_bit_cast(
-            T, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, ROWMASK, 0xF, true));
+            T, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, old), __builtin_bit_cast(int, v),
+                                           CTRL, ROWMASK, 0xF, true));
     }
 }
 template <int LANE, class T>
@@ -114,7 +118,9 @@ struct Lanes {
 template <class T>
 struct ScanMul {
     T F1, F2, F4, F8, FB15, FB31;  // forward, window products of the lanes' P
-    T G1, G2, G4, G8, XB;          // backward, window products of the lanes' Q
+    T G1, G2, G4, G8;              // backward, window products of the lanes' Q
+    T X16, X32, X48;               // backward across rows: nonzero only in the row
+                                   // that takes lane 16 / 32 / 48
     T FW, ZW;                      // two-wave groups: carries between the waves
 };
 
@@ -294,7 +300,15 @@ __device__ __forceinline__ void setup(const FdBatch& b, const Pde& d, const Lane
     sm.G2 = pr + 2 <= 15 ? lds_prod(lq, gl, gl + 1) : 0.0;
     sm.G4 = pr + 4 <= 15 ? lds_prod(lq, gl, gl + 3) : 0.0;
     sm.G8 = pr + 8 <= 15 ? lds_prod(lq, gl, gl + 7) : 0.0;
-    sm.XB = lds_prod(lq, gl, 16 * row16 + 15);
+    {
+        // product of Q from this lane to the end of its row, applied to the first lane of
+        // the next row; each readlane stage updates one row only
+        const double xb = lds_prod(lq, gl, 16 * row16 + 15);
+        const bool w64 = L == 64, w32 = L == 32;
+        sm.X16 = (w64 || w32) && wrow == 0 ? xb : 0.0;
+        sm.X32 = w64 && wrow == 1 ? xb : 0.0;
+        sm.X48 = (w64 && wrow == 2) || (w32 && wrow == 2) ? xb : 0.0;
+    }
     // two-wave groups: carry of wave 0's last lane into wave 1, and of wave 1's first
     // lane back into wave 0
     sm.FW = (W == 2 && wv == 1) ? lds_prod(qp, 64, gl) : 0.0;
@@ -326,9 +340,17 @@ __device__ __forceinline__ T fwd_carry(T c, const ScanMul<T>& sm, const Lanes& t
     c = fma(sm.F1, dpp<kRowShr + 1>(c), c);
     c = fma(sm.F2, dpp<kRowShr + 2>(c), c);
     c = fma(sm.F4, dpp<kRowShr + 4>(c), c);
-    c = fma(sm.F8, dpp<kRowShr + 8>(c), c);
-    if constexpr (L >= 32) c = fma(sm.FB15, dpp<kRowBcast15, 0xA>(c), c);
-    if constexpr (L == 64) c = fma(sm.FB31, dpp<kRowBcast31, 0xC>(c), c);
+    T s = dpp<kRowShr + 8>(c);
+    c = fma(sm.F8, s, c);
+    // rows outside the broadcast's mask keep s (finite) and have a zero multiplier
+    if constexpr (L >= 32) {
+        s = dpp<kRowBcast15, 0xA>(c, s);
+        c = fma(sm.FB15, s, c);
+    }
+    if constexpr (L == 64) {
+        s = dpp<kRowBcast31, 0xC>(c, s);
+        c = fma(sm.FB31, s, c);
+    }
     T fin;
     if constexpr (W == 2) {
         if (t.wv == 0 && t.lane == 63) xchg[0] = c;
@@ -358,18 +380,14 @@ __device__ __forceinline__ T bwd_carry(T cb, const ScanMul<T>& sm, const Lanes& 
     cb = fma(sm.G2, dpp<kRowShl + 2>(cb), cb);
     cb = fma(sm.G4, dpp<kRowShl + 4>(cb), cb);
     cb = fma(sm.G8, dpp<kRowShl + 8>(cb), cb);
-    const int wrow = t.wrow;
+    // across rows: the multipliers are zero outside the row each stage updates
     if constexpr (L == 64) {
-        T u = fma(sm.XB, readlane<48>(cb), cb);
-        cb = wrow == 2 ? u : cb;
-        u = fma(sm.XB, readlane<32>(cb), cb);
-        cb = wrow == 1 ? u : cb;
-        u = fma(sm.XB, readlane<16>(cb), cb);
-        cb = wrow == 0 ? u : cb;
+        cb = fma(sm.X48, readlane<48>(cb), cb);
+        cb = fma(sm.X32, readlane<32>(cb), cb);
+        cb = fma(sm.X16, readlane<16>(cb), cb);
     } else if constexpr (L == 32) {
-        const T u0 = fma(sm.XB, readlane<16>(cb), cb);
-        const T u2 = fma(sm.XB, readlane<48>(cb), cb);
-        cb = wrow == 0 ? u0 : (wrow == 2 ? u2 : cb);
+        cb = fma(sm.X16, readlane<16>(cb), cb);
+        cb = fma(sm.X48, readlane<48>(cb), cb);
     }
     T zin;
     if constexpr (W == 2) {
@@ -518,7 +536,8 @@ __device__ __forceinline__ void solve_group_mixed(const FdBatch& b, int group, d
         sm.F1 = (float)s64.F1; sm.F2 = (float)s64.F2; sm.F4 = (float)s64.F4;
         sm.F8 = (float)s64.F8; sm.FB15 = (float)s64.FB15; sm.FB31 = (float)s64.FB31;
         sm.G1 = (float)s64.G1; sm.G2 = (float)s64.G2; sm.G4 = (float)s64.G4;
-        sm.G8 = (float)s64.G8; sm.XB = (float)s64.XB; sm.FW = (float)s64.FW;
+        sm.G8 = (float)s64.G8;
+        sm.X16 = (float)s64.X16; sm.X32 = (float)s64.X32; sm.X48 = (float)s64.X48; sm.FW = (float)s64.FW;
         sm.ZW = (float)s64.ZW;
     }
     // diagonal of (I - B) is theta dt a0 on every real row (the stencils' weights sum

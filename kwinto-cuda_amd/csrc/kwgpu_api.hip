@@ -652,6 +652,78 @@ int kwgpu_fd1d_bs_price(kwgpu_ctx* c, const kwgpu_option* opts, size_t n,
     return collect(c, opts, n, p, st, prefix);
 }
 
+int kwgpu_fd1d_greeks(kwgpu_ctx* c, const kwgpu_option* opts, size_t n,
+                      const kwgpu_fd1d_params* p, double bump, kwgpu_greeks* out,
+                      int32_t* status) {
+#pragma clang fp contract(off)
+    constexpr size_t kSets = 9;  // as given, then s, z, t, r times (1 + bump), (1 - bump)
+    if (!c) return KWGPU_ERR_INVALID;
+    c->err.clear();
+    if (int rc = check_params(c, p)) return rc;
+    if (n == 0) return KWGPU_OK;
+    if (!opts || !out) return fail(c, KWGPU_ERR_INVALID, "kwgpu: null pointer");
+    if (!(bump > 0.0 && bump < 1.0)) return fail(c, KWGPU_ERR_INVALID, "kwgpu: bump not in (0, 1)");
+    if (n > 0x7fffffffu / kSets) return fail(c, KWGPU_ERR_INVALID, "kwgpu: too many options");
+    if (int rc = check_dims(c, p->x_dims, p->t_dims, n)) return rc;
+    if (hipError_t e = hipSetDevice(c->device); e != hipSuccess) return hip_fail(c, e, "hipSetDevice");
+    // option i's nine prices sit at kSets * i + k
+    std::vector<kwgpu_option> all(n * kSets);
+    std::vector<int32_t> xd, td;
+    kwgpu_fd1d_params p9 = *p;
+    if (p->x_dims) xd.resize(n * kSets);
+    if (p->t_dims) td.resize(n * kSets);
+    for (size_t i = 0; i < n; ++i) {
+        for (size_t k = 0; k < kSets; ++k) {
+            kwgpu_option a = opts[i];
+            if (k > 0) {
+                const double f = (k - 1) % 2 == 0 ? 1.0 + bump : 1.0 - bump;
+                switch ((k - 1) / 2) {
+                    case 0: a.s = f * a.s; break;
+                    case 1: a.z = f * a.z; break;
+                    case 2: a.t = f * a.t; break;
+                    default: a.r = f * a.r; break;
+                }
+            }
+            all[kSets * i + k] = a;
+            if (p->x_dims) xd[kSets * i + k] = p->x_dims[i];
+            if (p->t_dims) td[kSets * i + k] = p->t_dims[i];
+        }
+    }
+    if (p->x_dims) p9.x_dims = xd.data();
+    if (p->t_dims) p9.t_dims = td.data();
+    const size_t n9 = n * kSets;
+    if (int rc = fd_price_host(c, all.data(), n9, &p9, c->opts, c->prices, c->status)) return rc;
+    std::vector<double> v(n9);
+    std::vector<int32_t> st(n9);
+    hipError_t e;
+    if ((e = hipMemcpy(v.data(), c->prices.p, n9 * sizeof(double), hipMemcpyDeviceToHost)) ||
+        (e = hipMemcpy(st.data(), c->status.p, n9 * sizeof(int32_t), hipMemcpyDeviceToHost)))
+        return hip_fail(c, e, "kwgpu: download");
+    for (size_t i = 0; i < n; ++i) {
+        const double* w = &v[kSets * i];
+        const kwgpu_option& a = opts[i];
+        auto first = [&](int k, double x) { return (w[k] - w[k + 1]) / (2.0 * bump * x) + 0.0; };
+        const double bs = bump * a.s;
+        kwgpu_greeks g;
+        g.price = w[0];
+        g.delta = first(1, a.s);
+        g.gamma = (w[1] - 2.0 * w[0] + w[2]) / (bs * bs) + 0.0;
+        g.vega = first(3, a.z) / 100.0;
+        g.theta = first(5, a.t) / 360.0;
+        g.rho = first(7, a.r) / 100.0;
+        out[i] = g;
+        if (status) {
+            status[i] = KWGPU_STATUS_OK;
+            for (size_t k = 0; k < kSets; ++k)
+                if (st[kSets * i + k] != KWGPU_STATUS_OK) {
+                    status[i] = st[kSets * i + k];
+                    break;
+                }
+        }
+    }
+    return collect(c, all.data(), n9, &p9, st, "kwgpu_fd1d_greeks: ");
+}
+
 }  // extern "C"
 
 // ---------------------------------------------------------------------------------

@@ -25,7 +25,7 @@ LAYOUT_AUTO, LAYOUT_LANE_SOA, LAYOUT_WAVE_REG = 0, 1, 2
 EXPORTED_SYMBOLS = (
     "kwgpu_fd1d_params_default", "kwgpu_create", "kwgpu_destroy", "kwgpu_last_error",
     "kwgpu_fd1d_price", "kwgpu_fd1d_price_device", "kwgpu_bs_price", "kwgpu_bs_price_device",
-    "kwgpu_fd1d_bs_price", "kwgpu_synchronize", "kwgpu_stream",
+    "kwgpu_fd1d_bs_price", "kwgpu_fd1d_greeks", "kwgpu_synchronize", "kwgpu_stream",
     "kwgpu_surrogate_train_params_default", "kwgpu_surrogate_create", "kwgpu_surrogate_destroy",
     "kwgpu_surrogate_train", "kwgpu_surrogate_predict", "kwgpu_surrogate_price",
     "kwgpu_surrogate_get_params", "kwgpu_surrogate_set_params", "kwgpu_surrogate_loss_grad",
@@ -33,6 +33,7 @@ EXPORTED_SYMBOLS = (
     "kwgpu_fd1d_autotune", "kwgpu_tune_save", "kwgpu_tune_load",
 )
 SURROGATE_PARAMS = 199425
+GREEKS_DTYPE = np.dtype([(f, "<f8") for f in ("price", "delta", "gamma", "vega", "theta", "rho")])
 
 
 class SurrogateTrainParams(ctypes.Structure):
@@ -80,6 +81,8 @@ def lib():
         L.kwgpu_bs_price.argtypes = [P, P, SZ, P]
         L.kwgpu_bs_price_device.argtypes = [P, P, SZ, P, P]
         L.kwgpu_fd1d_bs_price.argtypes = [P, P, SZ, ctypes.POINTER(Fd1dParams), P, P]
+        L.kwgpu_fd1d_greeks.argtypes = [P, P, SZ, ctypes.POINTER(Fd1dParams), ctypes.c_double,
+                                        P, P]
         L.kwgpu_synchronize.argtypes = [P]
         L.kwgpu_stream.argtypes = [P]
         L.kwgpu_stream.restype = P
@@ -199,6 +202,22 @@ class Context:
         rc = lib().kwgpu_fd1d_bs_price(self.h, a.ctypes.data, n, ctypes.byref(p),
                                        prices.ctypes.data, status.ctypes.data)
         return prices, status, rc, (self.last_error if rc else "")
+
+    def fd1d_greeks(self, opts, x_dim=512, t_dim=512, bump=0.01, *, x_dims=None, t_dims=None,
+                    **kw):
+        """(greeks, status, rc, error): greeks is a structured array with fields
+        price, delta, gamma, vega, theta, rho (central bumps of relative size `bump`)."""
+        a = as_options(opts)
+        n = len(a)
+        out = np.full(n, np.nan, dtype=GREEKS_DTYPE)
+        status = np.zeros(n, np.int32)
+        xd = None if x_dims is None else np.ascontiguousarray(x_dims, np.int32)
+        td = None if t_dims is None else np.ascontiguousarray(t_dims, np.int32)
+        p = params(x_dim, t_dim, x_dims=None if xd is None else xd.ctypes.data,
+                   t_dims=None if td is None else td.ctypes.data, **kw)
+        rc = lib().kwgpu_fd1d_greeks(self.h, a.ctypes.data, n, ctypes.byref(p), bump,
+                                     out.ctypes.data, status.ctypes.data)
+        return out, status, rc, (self.last_error if rc else "")
 
     def bs_price(self, opts):
         a = as_options(opts)

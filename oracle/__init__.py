@@ -151,3 +151,55 @@ def ref_price(pricer, opts, x_dim=512, t_dim=512, density=0.25, scale=50.0):
     rc = ref_lib().kwref_price(pricer.encode(), a.ctypes.data, len(a), density, scale, t_dim,
                                x_dim, out.ctypes.data, err, 512)
     return out, (err.value.decode() if rc else "")
+
+
+# ---- Greeks by central bumps (test/portfolio.py:35-56) ----------------------------
+# delta = (V(s(1+b)) - V(s(1-b))) / (2 b s); gamma = (V(s(1+b)) - 2 V + V(s(1-b))) / (b s)^2;
+# vega, theta, rho are the first differences in z, t, r divided by 100, 360 and 100.
+
+GREEK_FIELDS = ("price", "delta", "gamma", "vega", "theta", "rho")
+BUMPED = ("s", "z", "t", "r")
+
+
+def bump_sets(opts, bump=0.01):
+    """The nine option arrays the bump formulas price: the options as given, then each
+    of s, z, t, r times (1 + bump) and times (1 - bump)."""
+    a = _opts(opts)
+    sets = [a.copy()]
+    for f in BUMPED:
+        for sgn in (1.0, -1.0):
+            b = a.copy()
+            b[f] = (1.0 + sgn * bump) * a[f]
+            sets.append(b)
+    return sets
+
+
+def greeks_from_prices(opts, v, bump=0.01):
+    """Greeks from the (9, n) prices of bump_sets, evaluated in the order the formulas
+    are written (a zero rate, volatility or expiry gives NaN where the script would
+    divide by zero)."""
+    a = _opts(opts)
+    v = np.asarray(v, np.float64)
+    out = np.zeros(len(a), dtype=[(f, "<f8") for f in GREEK_FIELDS])
+    with np.errstate(divide="ignore", invalid="ignore"):
+        def first(k, f):
+            return (v[k] - v[k + 1]) / (2.0 * bump * a[f]) + 0.0
+        out["price"] = v[0]
+        out["delta"] = first(1, "s")
+        out["gamma"] = (v[1] - 2.0 * v[0] + v[2]) / (bump * a["s"]) ** 2 + 0.0
+        out["vega"] = first(3, "z") / 100.0
+        out["theta"] = first(5, "t") / 360.0
+        out["rho"] = first(7, "r") / 100.0
+    return out
+
+
+def fd1d_greeks(opts, x_dim=512, t_dim=512, bump=0.01, density=0.25, scale=50.0, nthreads=0):
+    """(greeks, status, prices[9, n]) of the FD pricer by the bump formulas; status is the
+    first non-zero status of an option's nine prices."""
+    sets = bump_sets(opts, bump)
+    v, st = zip(*[fd1d_price_status(s, x_dim, t_dim, density, scale, nthreads) for s in sets])
+    v, st = np.stack(v), np.stack(st)
+    status = np.zeros(st.shape[1], np.int32)
+    for k in range(st.shape[0] - 1, -1, -1):  # the lowest-numbered failing price wins
+        status = np.where(st[k] != 0, st[k], status)
+    return greeks_from_prices(opts, v, bump), status, v

@@ -126,11 +126,60 @@ static void errorTest() {
     std::printf("errors done\n");
 }
 
+// Fd1d_Pricer::greeks against the bump formulas (test/portfolio.py:35-56) applied to
+// separate price() calls on the bumped options.
+static void greeksTest() {
+    std::vector<Option> assets;
+    for (const auto& t : testData()) assets.push_back(t.first);
+    Config config;
+    auto pricer = make_sPtr<Fd1d_Pricer>();
+    auto err = pricer->init(config);
+    EXPECT(err.empty(), "greeks init: %s", err.c_str());
+    if (!err.empty()) return;
+    std::vector<kwgpu_greeks> g;
+    const f64 b = 0.01;
+    err = pricer->greeks(assets, g, b);
+    EXPECT(err.empty() && g.size() == assets.size(), "greeks: %s", err.c_str());
+    if (!err.empty()) return;
+    std::vector<f64> v[9];
+    for (int k = 0; k < 9; ++k) {
+        auto a = assets;
+        for (auto& o : a) {
+            const f64 f = k == 0 ? 1.0 : (k - 1) % 2 == 0 ? 1.0 + b : 1.0 - b;
+            f64* x[4] = {&o.s, &o.z, &o.t, &o.r};
+            if (k > 0) *x[(k - 1) / 2] = f * *x[(k - 1) / 2];
+        }
+        err = pricer->price(a, v[k]);
+        EXPECT(err.empty(), "bumped price %d: %s", k, err.c_str());
+        if (!err.empty()) return;
+    }
+    for (size_t i = 0; i < assets.size(); ++i) {
+        const auto& o = assets[i];
+        auto d = [&](int k, f64 x) { return (v[k][i] - v[k + 1][i]) / (2 * b * x); };
+        const f64 want[6] = {v[0][i], d(1, o.s), (v[1][i] - 2 * v[0][i] + v[2][i]) / ((b * o.s) * (b * o.s)),
+                             d(3, o.z) / 100, d(5, o.t) / 360, d(7, o.r) / 100};
+        const f64 got[6] = {g[i].price, g[i].delta, g[i].gamma, g[i].vega, g[i].theta, g[i].rho};
+        for (int j = 0; j < 6; ++j)
+            EXPECT(std::abs(want[j] - got[j]) <= 1e-9 * std::max(1.0, std::abs(want[j])),
+                   "greek %d of #%zu want %.12g got %.12g", j, i, want[j], got[j]);
+        // European puts and calls: Black-Scholes delta within the grid's accuracy
+        if (!o.e) {
+            const f64 sq = o.z * std::sqrt(o.t);
+            const f64 d1 = (std::log(o.s / o.k) + (o.r - o.q + 0.5 * o.z * o.z) * o.t) / sq;
+            const f64 delta = o.w * std::exp(-o.q * o.t) * 0.5 * std::erfc(-o.w * d1 / std::sqrt(2.));
+            EXPECT(std::abs(delta - g[i].delta) <= 2e-3, "BS delta #%zu want %.6f got %.6f", i,
+                   delta, g[i].delta);
+        }
+    }
+    std::printf("greeks done\n");
+}
+
 int main(int argc, char** argv) {
     pricerTest("BlackScholes", "BS", true);
     pricerTest("Fd1d", "FD1D", false);
     pricerTest("Fd1dBlackScholes", "FD1D-BS", false);
     errorTest();
+    greeksTest();
     if (argc > 1) portfolioTest(argv[1]);
     std::printf("%d failed\n", g_failed);
     return g_failed;

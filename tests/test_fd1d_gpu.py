@@ -80,6 +80,48 @@ def test_fd1d_bs_plain_512(ctx):
     assert np.max(np.abs(ctx.bs_price(e) - c["ref_bs_euro"])) <= 1e-12
 
 
+@pytest.mark.parametrize("precision", ["fp64", "uneven"])
+def test_greeks_match_oracle(ctx, precision):
+    """Bump Greeks (test/portfolio.py:35-56) through the C-ABI against the oracle's: each
+    Greek is a fixed linear combination of nine prices, so its tolerance is that
+    combination applied to the price tolerance 1e-8 max(1, |V|)."""
+    _say("greeks", precision)
+    o = W.plain(96, 17)
+    b = 0.01
+    if precision == "uneven":
+        xd = np.where(np.arange(len(o)) % 2, 256, 512).astype(np.int32)
+        td = np.where(np.arange(len(o)) % 3, 128, 200).astype(np.int32)
+        g, st, rc, err = ctx.fd1d_greeks(o, 0, 0, b, x_dims=xd, t_dims=td)
+        sets = oracle.bump_sets(o, b)
+        v = np.stack([oracle.fd1d_price_uneven(s_, xd, td)[0] for s_ in sets])
+        ref = oracle.greeks_from_prices(o, v, b)
+    else:
+        g, st, rc, err = ctx.fd1d_greeks(o, 512, 512, b)
+        ref, _, v = oracle.fd1d_greeks(o, 512, 512, b)
+    assert rc == kwgpu.OK, err
+    assert np.all(st == 0)
+    ptol = EPS * np.maximum(1.0, np.abs(v).max(axis=0))
+    tol = {"price": ptol,
+           "delta": 2 * ptol / (2 * b * o["s"]),
+           "gamma": 4 * ptol / (b * o["s"]) ** 2,
+           "vega": 2 * ptol / (2 * b * o["z"]) / 100,
+           "theta": 2 * ptol / (2 * b * o["t"]) / 360,
+           "rho": 2 * ptol / (2 * b * o["r"]) / 100}
+    for f, tl in tol.items():
+        assert np.all(np.abs(g[f] - ref[f]) <= tl), f
+
+
+def test_greeks_bad_bump_and_range(ctx):
+    o = W.plain(4, 18)
+    _, _, rc, err = ctx.fd1d_greeks(o, 256, 64, 0.0)
+    assert rc == kwgpu.ERR_INVALID and "bump" in err
+    # a spot at the grid's upper end: the up-bump leaves the grid
+    o["s"] = o["k"] * np.exp(50 * o["z"] * np.sqrt(o["t"])) * 0.999
+    _, st, rc, err = ctx.fd1d_greeks(o, 256, 64, 0.01)
+    assert rc == kwgpu.ERR_RANGE and "not in range" in err
+    assert np.all(st == kwgpu.STATUS_OUT_OF_GRID)
+
+
 def test_qdfp_portfolio(ctx):
     """test/kwPortfolio_test.cpp:29-58 through the GPU: 6000 puts (600 PDEs after the
     reference's compression) within 0.005 of QuantLib, and at parity with the
@@ -188,7 +230,8 @@ def test_theta_general_matches_reference_at_half(ctx):
 
 def test_device_path_full_size_properties(ctx):
     """32768 options, 1024 x 1024, device-resident: head equals the golden vectors,
-    American >= European >= 0, American >= intrinsic, bitwise deterministic."""
+    American >= European >= 0, American >= K x the payoff interpolated between the nodes
+    around log(S/K), bitwise deterministic."""
     import torch
     _say("full size 32768 x 1024 x 1024")
     n = 32768
@@ -214,8 +257,42 @@ def test_device_path_full_size_properties(ctx):
     pe = d_p.cpu().numpy()
     assert np.all(pe >= -1e-12)
     assert np.all(p >= pe - 1e-12)
+    # projection holds at the nodes j <= N-2 and the price interpolates linearly between
+    # the two nodes around log(S/K), so it is bounded below by K times the payoff
+    # interpolated the same way (not by the intrinsic value: the put payoff 1 - e^x is
+    # concave, so the chord lies below it)
+    lo = K_times_interpolated_payoff(o, 1024)
+    assert np.all(p >= lo - 1e-10 * o["k"])
     intrinsic = np.maximum(0, o["w"] * (o["s"] - o["k"]))
-    assert np.all(p >= intrinsic - 1e-9)
+    assert np.all(p >= intrinsic - 1e-3 * o["k"])
+
+
+def K_times_interpolated_payoff(o, x_dim, density=0.25, scale=50.0):
+    """K * (linear interpolation of the payoff between the grid nodes around log(S/K)),
+    with the grid x_j = density sinh(y_min (1 - j/(N-1)) + y_max j/(N-1)),
+    y = asinh(-+scale sigma sqrt(T) / density) and the first node x_j >= log(S/K)
+    (src/Pricer/kwFd1d.cpp:105-139, src/Math/kwFd1d.cpp:139-158)."""
+    xs = np.log(o["s"] / o["k"])
+    half = scale * o["z"] * np.sqrt(o["t"])
+    y0, y1 = np.arcsinh(-half / density), np.arcsinh(half / density)
+
+    def node(j):
+        u = j / (x_dim - 1.0)
+        return density * np.sinh(y0 * (1.0 - u) + y1 * u)
+
+    j = np.ceil((np.arcsinh(xs / density) - y0) / (y1 - y0) * (x_dim - 1)).astype(np.int64)
+    j = np.clip(j, 1, x_dim - 1)
+    for _ in range(2):  # settle rounding at a node
+        j = np.where((j > 1) & (node(j - 1) >= xs), j - 1, j)
+        j = np.where((j < x_dim - 1) & (node(j) < xs), j + 1, j)
+    xl, xr = node(j - 1), node(j)
+    assert np.all((xl < xs) & (xs <= xr))
+
+    def payoff(x):
+        return np.maximum(0.0, o["w"] * (np.exp(x) - 1.0))
+
+    f = (xs - xl) / (xr - xl)
+    return o["k"] * (payoff(xl) + (payoff(xr) - payoff(xl)) * f)
 
 
 # ---- FP32-mixed: within 1e-4 of the FP64 reference -------------------------------

@@ -204,6 +204,67 @@ def test_american_ge_european_and_monotone_in_spot():
         assert np.all(d >= -1e-12) if o["w"][9 * i] > 0 else np.all(d <= 1e-12)
 
 
+def _bs_greeks(o):
+    """Closed-form Black-Scholes Greeks in the bump script's units: vega and rho per
+    1%, theta as dV/dT per day of 360 (test/portfolio.py:35-56)."""
+    from scipy.special import ndtr
+    s, k, t, z, r, q = (o[f] for f in ("s", "k", "t", "z", "r", "q"))
+    w = o["w"].astype(np.float64)
+    sq = z * np.sqrt(t)
+    d1 = (np.log(s / k) + (r - q + 0.5 * z * z) * t) / sq
+    d2 = d1 - sq
+    pdf = np.exp(-0.5 * d1 * d1) / math.sqrt(2 * math.pi)
+    eq, er = np.exp(-q * t), np.exp(-r * t)
+    return {
+        "price": w * (s * eq * ndtr(w * d1) - k * er * ndtr(w * d2)),
+        "delta": w * eq * ndtr(w * d1),
+        "gamma": eq * pdf / (s * sq),
+        "vega": s * eq * pdf * np.sqrt(t) / 100,
+        "theta": (s * eq * pdf * z / (2 * np.sqrt(t)) + w * r * k * er * ndtr(w * d2)
+                  - w * q * s * eq * ndtr(w * d1)) / 360,
+        "rho": w * k * t * er * ndtr(w * d2) / 100,
+    }
+
+
+def test_greeks_of_european_fd_match_black_scholes():
+    """The bump Greeks of the FD pricer on European options against the closed forms:
+    agreement to the grid's discretisation pins signs, factors and bump bookkeeping."""
+    rng = np.random.default_rng(0)
+    n = 12
+    o = np.zeros(n, oracle.OPTION_DTYPE)
+    o["s"] = 100
+    o["k"] = rng.uniform(80, 120, n)
+    o["t"] = rng.uniform(0.25, 2, n)
+    o["z"] = rng.uniform(0.15, 0.45, n)
+    o["r"] = rng.uniform(0.01, 0.08, n)
+    o["q"] = rng.uniform(0, 0.05, n)
+    o["w"] = np.where(np.arange(n) % 2, 1, -1)
+    g, st, _ = oracle.fd1d_greeks(o, 1024, 1024)
+    assert np.all(st == 0)
+    ref = _bs_greeks(o)
+    tol = {"price": 5e-4, "delta": 5e-4, "gamma": 5e-4, "vega": 5e-4, "theta": 5e-5,
+           "rho": 5e-5}
+    for f, tl in tol.items():
+        assert np.max(np.abs(g[f] - ref[f])) < tl, f
+
+
+def test_american_put_greeks_signs():
+    o = np.repeat(W.plain(4, 25), 5)
+    o["w"] = -1
+    o["s"] = np.tile([60.0, 80.0, 100.0, 120.0, 140.0], 4)
+    g, st, _ = oracle.fd1d_greeks(o, 512, 512)
+    assert np.all(st == 0)
+    # deep in the exercise region the bumped prices lie on chords of the concave payoff,
+    # which leaves gamma, vega and theta slightly negative (a few 1e-4 at 512 points)
+    assert np.all((g["delta"] >= -1 - 1e-6) & (g["delta"] <= 1e-12))
+    assert np.all(g["gamma"] >= -1e-4)
+    assert np.all(g["vega"] >= -1e-3) and np.all(g["theta"] >= -1e-4)
+    assert np.all(g["rho"] <= 1e-9)
+    live = o["s"] >= 80  # outside the exercise region the signs are strict
+    assert np.all(g["gamma"][live] > 0) and np.all(g["vega"][live] > 0)
+    assert np.all(g["theta"][live] > 0) and np.all(g["rho"][live] < 0)
+
+
 # ---- planted mistakes -----------------------------------------------------------
 
 PLANTS = {

@@ -194,8 +194,11 @@ void* kwgpu_stream(kwgpu_ctx* ctx);
  * of the FD pricer above and uses it as its source of labels and as its fallback).
  *
  * An ensemble of MLPs 6 -> 256 -> 256 -> 256 -> 256 -> 1 with SiLU predicts
- * y = asinh(premium / (K * 1e-5)) / 4, where premium = FD American price - Black-Scholes
- * European price, from features (log(S/K), T, sigma, r, q, w) normalised to the box
+ * y = 100 (premium - baseline) / K, where premium = FD American price - Black-Scholes
+ * European price and baseline is the Barone-Adesi & Whaley quadratic approximation of
+ * that premium (kwgpu_surrogate_baseline; a control variate that leaves the network
+ * about a tenth of the premium to learn), from features (log(S/K), T, sigma, r, q, w)
+ * normalised to the box
  * K/S in [0.4, 2], T in [1/52, 5], sigma in [0.1, 0.6], r in [0, 0.1], q in [0, 0.08].
  * The three 256x256 layers run on BF16 MFMA with FP32 accumulation (forward and
  * backward); the 6-wide input layer and the 1-wide output layer run in FP32; Adam
@@ -257,6 +260,11 @@ int kwgpu_surrogate_set_params(kwgpu_ctx* ctx, kwgpu_surrogate* s, int32_t membe
 int kwgpu_surrogate_loss_grad(kwgpu_ctx* ctx, kwgpu_surrogate* s, int32_t member,
                               const float* host_features, const float* host_targets, size_t n,
                               float* host_loss, float* host_grad);
+
+/* The analytic baseline premium (Barone-Adesi & Whaley, continuous dividend yield) of
+ * each option, host memory; price units. */
+int kwgpu_surrogate_baseline(kwgpu_ctx* ctx, const kwgpu_option* host_opts, size_t n,
+                             double* host_out);
 
 /* Normalised features (n x 6) of options, host memory, as the network sees them. */
 int kwgpu_surrogate_features(kwgpu_ctx* ctx, const kwgpu_option* host_opts, size_t n,

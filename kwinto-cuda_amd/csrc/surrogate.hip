@@ -35,11 +35,57 @@ constexpr int kW1 = 0, kB1 = kW1 + H * F, kW2 = kB1 + H, kB2 = kW2 + H * H, kW3 
               kB3 = kW3 + H * H, kW4 = kB3 + H, kB4 = kW4 + H * H, kW5 = kB4 + H, kB5 = kW5 + H,
               kP = kB5 + 1;
 static_assert(kP == KWGPU_SURROGATE_PARAMS, "parameter layout");
-constexpr double kS0 = 1e-5;     // premium per unit strike where the target turns logarithmic
-constexpr double kYScale = 4.0;  // target divisor, keeps y within about [-1, 3]
+// Target: y = kYScale (premium - baw_premium) / K, the part of the early-exercise premium
+// that the quadratic approximation below misses, per unit strike (about 0.3 rms).
+constexpr double kYScale = 100.0;
+constexpr uint32_t kFileVersion = 2;  // saved parameters belong to this target
 
-__host__ __device__ inline double premium_of(double y, double k) {
-    return k * kS0 * sinh(kYScale * y);
+// Barone-Adesi & Whaley (1987) quadratic approximation of the early-exercise premium
+// (American minus European Black-Scholes) with a continuous dividend yield q. With
+// M = 2r / sigma^2, N = 2 (r - q) / sigma^2, k(T) = 1 - e^{-rT},
+// q_w = (-(N - 1) + w sqrt((N - 1)^2 + 4 M / k(T))) / 2, the critical spot S* solves
+//     w (S* - K) = E(S*) + w (1 - e^{-qT} Phi(w d1(S*))) S* / q_w,
+// and the premium is A (S / S*)^{q_w} with A = w (S* / q_w)(1 - e^{-qT} Phi(w d1(S*)))
+// on the continuation side of S*, w (S - K) - E(S) beyond it. S* by bisection in log S
+// over [K/1000, K] (puts) or [K, 10^4 K] (calls); the premium is 0 where early exercise
+// never pays (calls with q <= 0, puts with r <= 0). A control variate only: the network
+// learns the remainder, so its approximation error is harmless.
+__host__ __device__ inline double baw_premium(const kwgpu_option& o) {
+    const double s = o.s, k = o.k, t = o.t, z = o.z, r = o.r, q = o.q;
+    const double w = o.w < 0 ? -1.0 : 1.0;
+    if (!(s > 0 && k > 0 && t > 0 && z > 0)) return 0.0;
+    if ((w > 0 && !(q > 0)) || (w < 0 && !(r > 0))) return 0.0;
+    const double z2 = z * z;
+    const double mk = 2 * r / (z2 * -expm1(-r * t));
+    const double nn = 2 * (r - q) / z2;
+    const double qw = 0.5 * (-(nn - 1) + w * sqrt((nn - 1) * (nn - 1) + 4 * mk));
+    const double sq = z * sqrt(t), eq = exp(-q * t), er = exp(-r * t);
+    const double c = 0.70710678118654752440;  // 1 / sqrt(2)
+    auto euro = [&](double S, double& nd1) {  // European price; nd1 = Phi(w d1)
+        const double d1 = (log(S / k) + (r - q + 0.5 * z2) * t) / sq;
+        nd1 = 0.5 * erfc(-w * d1 * c);
+        return w * (S * eq * nd1 - k * er * 0.5 * erfc(-w * (d1 - sq) * c));
+    };
+    double lo = w < 0 ? log(k * 1e-3) : log(k), hi = w < 0 ? log(k) : log(k * 1e4);
+    for (int it = 0; it < 64; ++it) {
+        const double mid = 0.5 * (lo + hi), S = exp(mid);
+        double nd1;
+        const double g = w * (S - k) - euro(S, nd1) - w * (1 - eq * nd1) * S / qw;
+        if (w < 0 ? g > 0 : g < 0)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    const double ss = exp(0.5 * (lo + hi));
+    double nd1s, nd1;
+    euro(ss, nd1s);
+    const double e = euro(s, nd1);
+    if (w < 0 ? s > ss : s < ss) return w * (ss / qw) * (1 - eq * nd1s) * pow(s / ss, qw);
+    return w * (s - k) - e;
+}
+
+__host__ __device__ inline double premium_of(double y, double k, double baw) {
+    return baw + k * y / kYScale;
 }
 
 // Training box, as (centre, half-width) of each raw feature.
@@ -80,7 +126,7 @@ __global__ void features_kernel(const kwgpu_option* opts, size_t n, const double
     float f[F];
     const bool in = option_features(o, f);
     for (int k = 0; k < F; ++k) feat[i * F + k] = f[k];
-    if (target) target[i] = (float)(asinh(premium[i] / (o.k * kS0)) / kYScale);
+    if (target) target[i] = (float)((premium[i] - baw_premium(o)) * kYScale / o.k);
     if (in_box) in_box[i] = in ? 1 : 0;
 }
 
@@ -91,6 +137,11 @@ __global__ void gather_kernel(const float* feat, const float* target, const uint
     const uint32_t s = idx[i];
     for (int k = 0; k < F; ++k) bfeat[i * F + k] = feat[(size_t)s * F + k];
     btarget[i] = target[s];
+}
+
+__global__ void baseline_kernel(const kwgpu_option* opts, size_t n, double* out) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = baw_premium(opts[i]);
 }
 
 // ---- layers ----------------------------------------------------------------------
@@ -301,9 +352,10 @@ __global__ void combine_kernel(const kwgpu_option* opts, const float* y, int mem
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double k = opts[i].k;
+    const double baw = baw_premium(opts[i]);
     double s = 0, lo = 1e300, hi = -1e300;
     for (int m = 0; m < members; ++m) {
-        const double v = premium_of(y[m * ld + i], k);
+        const double v = premium_of(y[m * ld + i], k, baw);
         s += v;
         lo = fmin(lo, v);
         hi = fmax(hi, v);
@@ -583,6 +635,32 @@ int kwgpu_surrogate_features(kwgpu_ctx* ctx, const kwgpu_option* opts, size_t n,
     return KWGPU_OK;
 }
 
+int kwgpu_surrogate_baseline(kwgpu_ctx* ctx, const kwgpu_option* opts, size_t n, double* out) {
+    if (!opts || !out) return kwgpu::ctx_fail(ctx, KWGPU_ERR_INVALID, "kwgpu_surrogate_baseline: bad argument");
+    if (n == 0) return KWGPU_OK;
+    hipStream_t st = kwgpu::ctx_stream(ctx);
+    kwgpu_option* d_o = nullptr;
+    double* d_v = nullptr;
+    int rc = KWGPU_OK;
+    if (hipMalloc(&d_o, n * sizeof(kwgpu_option)) || hipMalloc(&d_v, n * sizeof(double))) {
+        rc = kwgpu::ctx_fail(ctx, KWGPU_ERR_NOMEM, "kwgpu_surrogate_baseline: allocation");
+    } else {
+        hipError_t e = hipMemcpyAsync(d_o, opts, n * sizeof(kwgpu_option), hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(baseline_kernel, dim3(nblk(n, 256)), dim3(256), 0, st, d_o, n, d_v);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(out, d_v, n * sizeof(double), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess)
+            rc = kwgpu::ctx_fail(ctx, KWGPU_ERR_HIP, "kwgpu_surrogate_baseline: HIP error");
+    }
+    (void)hipFree(d_o);
+    (void)hipFree(d_v);
+    return rc;
+}
+
 int kwgpu_surrogate_get_params(kwgpu_ctx* ctx, const kwgpu_surrogate* s, int32_t member,
                                float* host) {
     if (!s || member < 0 || member >= s->members || !host)
@@ -802,7 +880,8 @@ int kwgpu_surrogate_save(kwgpu_ctx* ctx, const kwgpu_surrogate* s, const char* p
     if (!s || !path) return kwgpu::ctx_fail(ctx, KWGPU_ERR_INVALID, "kwgpu_surrogate_save: bad argument");
     FILE* f = std::fopen(path, "wb");
     if (!f) return kwgpu::ctx_fail(ctx, KWGPU_ERR_INVALID, std::string("cannot write ") + path);
-    const uint32_t hdr[4] = {0x4b575355u /* "KWSU" */, 1u, (uint32_t)s->members, (uint32_t)kP};
+    const uint32_t hdr[4] = {0x4b575355u /* "KWSU" */, kFileVersion, (uint32_t)s->members,
+                             (uint32_t)kP};
     std::fwrite(hdr, sizeof(hdr), 1, f);
     std::vector<float> w(kP);
     for (int m = 0; m < s->members; ++m) {
@@ -821,7 +900,7 @@ int kwgpu_surrogate_load(kwgpu_ctx* ctx, const char* path, kwgpu_surrogate** out
     FILE* f = std::fopen(path, "rb");
     if (!f) return kwgpu::ctx_fail(ctx, KWGPU_ERR_INVALID, std::string("cannot read ") + path);
     uint32_t hdr[4] = {};
-    if (std::fread(hdr, sizeof(hdr), 1, f) != 1 || hdr[0] != 0x4b575355u || hdr[1] != 1u ||
+    if (std::fread(hdr, sizeof(hdr), 1, f) != 1 || hdr[0] != 0x4b575355u || hdr[1] != kFileVersion ||
         hdr[3] != (uint32_t)kP || hdr[2] < 1 || hdr[2] > 16) {
         std::fclose(f);
         return kwgpu::ctx_fail(ctx, KWGPU_ERR_INVALID, "kwgpu_surrogate_load: not a surrogate file");

@@ -29,7 +29,7 @@ EXPORTED_SYMBOLS = (
     "kwgpu_surrogate_train_params_default", "kwgpu_surrogate_create", "kwgpu_surrogate_destroy",
     "kwgpu_surrogate_train", "kwgpu_surrogate_predict", "kwgpu_surrogate_price",
     "kwgpu_surrogate_get_params", "kwgpu_surrogate_set_params", "kwgpu_surrogate_loss_grad",
-    "kwgpu_surrogate_features", "kwgpu_surrogate_save", "kwgpu_surrogate_load",
+    "kwgpu_surrogate_features", "kwgpu_surrogate_baseline", "kwgpu_surrogate_save", "kwgpu_surrogate_load",
     "kwgpu_fd1d_autotune", "kwgpu_tune_save", "kwgpu_tune_load",
 )
 SURROGATE_PARAMS = 199425
@@ -100,6 +100,7 @@ def lib():
         L.kwgpu_surrogate_set_params.argtypes = [P, P, ctypes.c_int32, P]
         L.kwgpu_surrogate_loss_grad.argtypes = [P, P, ctypes.c_int32, P, P, SZ, P, P]
         L.kwgpu_surrogate_features.argtypes = [P, P, SZ, P]
+        L.kwgpu_surrogate_baseline.argtypes = [P, P, SZ, P]
         L.kwgpu_surrogate_save.argtypes = [P, P, ctypes.c_char_p]
         L.kwgpu_surrogate_load.argtypes = [P, ctypes.c_char_p, ctypes.POINTER(P)]
         L.kwgpu_fd1d_autotune.argtypes = [P, P, SZ, ctypes.POINTER(Fd1dParams), P, P,
@@ -356,4 +357,15 @@ def surrogate_features(opts) -> np.ndarray:
     rc = lib().kwgpu_surrogate_features(None, a.ctypes.data, len(a), out.ctypes.data)
     if rc != OK:
         raise KwgpuError(rc, "kwgpu_surrogate_features failed")
+    return out
+
+
+def surrogate_baseline(ctx: Context, opts) -> np.ndarray:
+    """The analytic baseline premium (Barone-Adesi & Whaley) the network's target is
+    taken relative to, computed on the device."""
+    a = as_options(opts)
+    out = np.full(len(a), np.nan)
+    rc = lib().kwgpu_surrogate_baseline(ctx.h, a.ctypes.data, len(a), out.ctypes.data)
+    if rc != OK:
+        raise KwgpuError(rc, ctx.last_error)
     return out

@@ -265,6 +265,46 @@ def test_american_put_greeks_signs():
     assert np.all(g["theta"][live] > 0) and np.all(g["rho"][live] < 0)
 
 
+def test_baw_baseline_tracks_the_fd_american_price():
+    """The surrogate's analytic baseline: BS + BAW premium within the approximation's
+    known accuracy of the FD American price (a wrong sign or root gives errors of whole
+    price units), value matching at the critical spot, and no premium where early
+    exercise never pays."""
+    from baw_ref import baw_premium, critical_spot, european
+    rng = np.random.default_rng(7)
+    n = 48
+    o = np.zeros(n, oracle.OPTION_DTYPE)
+    o["s"] = 100
+    o["k"] = rng.uniform(85, 115, n)
+    o["t"] = rng.uniform(0.25, 1.0, n)
+    o["z"] = rng.uniform(0.2, 0.4, n)
+    o["r"] = rng.uniform(0.03, 0.08, n)
+    o["q"] = rng.uniform(0.02, 0.08, n)
+    o["w"] = np.where(np.arange(n) % 2, 1, -1)
+    o["e"] = 1
+    fd, err = oracle.fd1d_price(o, 1024, 1024)
+    assert err == ""
+    e = o.copy()
+    e["e"] = 0
+    eu = oracle.bs_price(e)
+    prem, left = fd - eu, fd - eu - baw_premium(o)
+    assert np.max(np.abs(left)) < 0.15 and np.median(np.abs(left)) < 0.02
+    # it removes most of the premium (here about 80%) from what the network must learn
+    assert np.sqrt(np.mean(left ** 2)) < 0.3 * np.sqrt(np.mean(prem ** 2))
+    assert np.all(baw_premium(o) >= 0)
+    # at S = S* the continuation formula meets the exercise value
+    ss, _ = critical_spot(o)
+    at = o.copy()
+    at["s"] = ss * (1 + 1e-12)
+    eur, _ = european(at)
+    assert np.allclose(baw_premium(at) + eur, np.where(o["w"] < 0, o["k"] - ss, ss - o["k"]),
+                       rtol=0, atol=1e-6 * o["k"])
+    z = o.copy()
+    z["q"] = np.where(z["w"] > 0, 0.0, z["q"])
+    z["r"] = np.where(z["w"] < 0, 0.0, z["r"])
+    assert np.all(baw_premium(z) == 0)
+
+
 # ---- planted mistakes -----------------------------------------------------------
 
 PLANTS = {

@@ -143,6 +143,20 @@ def test_gradient_close_to_fp32_autograd(sur):
     assert cos > 0.995, cos
 
 
+def test_baseline_matches_numpy_restatement(ctx):
+    """The device's Barone-Adesi & Whaley premium against tests/baw_ref.py, including
+    the no-early-exercise cases (puts at r = 0, calls at q = 0)."""
+    from baw_ref import baw_premium
+    o = W.surrogate_train(4096, 12)
+    o[:64]["r"] = 0.0
+    o[64:128]["q"] = 0.0
+    got = kwgpu.surrogate_baseline(ctx, o)
+    ref = baw_premium(o)
+    assert np.all(np.isfinite(got))
+    assert np.max(np.abs(got - ref) / o["k"]) < 1e-9
+    assert np.all(got[:64][o[:64]["w"] < 0] == 0) and np.all(got[64:128][o[64:128]["w"] > 0] == 0)
+
+
 def _fd_premium(ctx, opts, x_dim, t_dim, precision=kwgpu.FP64):
     """FD American price minus BS European price, both on the GPU."""
     import torch

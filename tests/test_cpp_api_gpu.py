@@ -35,3 +35,23 @@ def test_reference_test_cases_through_cpp_api(tmp_path):
     print(r.stdout[-3000:], r.stderr[-2000:], flush=True)
     assert r.returncode == 0, r.stdout[-3000:]
     assert "0 failed" in r.stdout
+
+
+CLI = os.path.join(PKG, "bin", "kwinto")
+
+
+def test_cli_prices_the_qdfp_portfolio(tmp_path):
+    """`kwinto price` (src/kwinto.cpp:40-83) on the QuantLib fixture: FD1D at 512 x 512
+    within the reference test's 0.005 of QuantLib, statistics in the reference's format."""
+    assert os.path.exists(CLI), "build() first"
+    o, c = load_golden("portfolio_qdfp")
+    csv = tmp_path / "portfolio_qdfp.csv"
+    _write_reference_csv(csv, o, [float(x) for x in c["want"]])
+    r = subprocess.run([CLI, "price", "-p", "FD1D", "-t", "512", "-x", "512", str(csv)],
+                       capture_output=True, text=True, timeout=300)
+    print(r.stdout[-2000:], r.stderr[-2000:], flush=True)
+    assert r.returncode == 0
+    assert "Assets : 6000" in r.stdout and "Price Statistics" in r.stdout
+    mae = float(r.stdout.split("MAE :")[1].split()[0])
+    assert mae <= 0.005
+    assert "total : 6000 options" in r.stdout

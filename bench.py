@@ -74,6 +74,27 @@ def read_traffic():
     return float(m.group(1)) if m else None
 
 
+def error_by_class(opts, d):
+    """Max abs error per (put | call) x (T < 1 | T >= 1)."""
+    out = {}
+    for w, wn in ((-1, "put"), (1, "call")):
+        for lo, hi, tn in ((0.0, 1.0, "T<1"), (1.0, np.inf, "T>=1")):
+            m = (opts["w"] == w) & (opts["t"] >= lo) & (opts["t"] < hi)
+            out[f"{wn} {tn}"] = float(d[m].max()) if m.any() else None
+    return out
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(opts, sample, threads):
     import oracle
     sub = opts[:sample]
@@ -85,7 +106,8 @@ def cpu_baseline(opts, sample, threads):
         raise RuntimeError(err)
     return {"value": sample / dt, "unit": "options/s", "cores": threads, "kind": "port",
             "sample": f"first {sample} options of the same workload, {X_DIM}x{T_DIM} FP64, "
-                      f"{threads} threads, one job per PDE ({dt:.2f} s)"}
+                      f"{threads} threads, one job per PDE ({dt:.2f} s); host {cpu_model()}, "
+                      f"{os.cpu_count()} logical CPUs visible"}
 
 
 def bench_uneven(ctx, torch, stream, n, steps, log):
@@ -213,6 +235,7 @@ def main():
             "kernel_ms": float(np.mean(kms_mx)),
             "max_abs_err_vs_fp64_fd": float(d.max()),
             "rms_abs_err_vs_fp64_fd": float(np.sqrt(np.mean(d * d))),
+            "max_abs_err_by_class": error_by_class(opts, d),
             "speedup_vs_fp64": ms_per_step / ms_mx}
         print(f"bench: FP32-mixed {ms_mx:.3f} ms/step, max err {d.max():.2e}", file=sys.stderr,
               flush=True)

@@ -119,8 +119,9 @@ template <class T>
 struct ScanMul {
     T F1, F2, F4, F8, FB15, FB31;  // forward, window products of the lanes' P
     T G1, G2, G4, G8;              // backward, window products of the lanes' Q
-    T X16, X32, X48;               // backward across rows: nonzero only in the row
-                                   // that takes lane 16 / 32 / 48
+    T X16, X32, X48;               // backward across rows: L = 64, nonzero only in the
+                                   // row that takes lane 16 / 32 / 48; L = 32, X16 is
+                                   // nonzero in rows 0 and 2 (each segment's first row)
     T FW, ZW;                      // two-wave groups: carries between the waves
 };
 
@@ -305,9 +306,9 @@ __device__ __forceinline__ void setup(const FdBatch& b, const Pde& d, const Lane
         // the next row; each readlane stage updates one row only
         const double xb = lds_prod(lq, gl, 16 * row16 + 15);
         const bool w64 = L == 64, w32 = L == 32;
-        sm.X16 = (w64 || w32) && wrow == 0 ? xb : 0.0;
+        sm.X16 = (w64 && wrow == 0) || (w32 && (wrow & 1) == 0) ? xb : 0.0;
         sm.X32 = w64 && wrow == 1 ? xb : 0.0;
-        sm.X48 = (w64 && wrow == 2) || (w32 && wrow == 2) ? xb : 0.0;
+        sm.X48 = w64 && wrow == 2 ? xb : 0.0;
     }
     // two-wave groups: carry of wave 0's last lane into wave 1, and of wave 1's first
     // lane back into wave 0
@@ -380,14 +381,16 @@ __device__ __forceinline__ T bwd_carry(T cb, const ScanMul<T>& sm, const Lanes& 
     cb = fma(sm.G2, dpp<kRowShl + 2>(cb), cb);
     cb = fma(sm.G4, dpp<kRowShl + 4>(cb), cb);
     cb = fma(sm.G8, dpp<kRowShl + 8>(cb), cb);
-    // across rows: the multipliers are zero outside the row each stage updates
+    // Across rows. A zero multiplier stands in for a select only inside one option:
+    // 0 * x is NaN when x is not finite, so no lane may read another option's value.
     if constexpr (L == 64) {
         cb = fma(sm.X48, readlane<48>(cb), cb);
         cb = fma(sm.X32, readlane<32>(cb), cb);
         cb = fma(sm.X16, readlane<16>(cb), cb);
     } else if constexpr (L == 32) {
-        cb = fma(sm.X16, readlane<16>(cb), cb);
-        cb = fma(sm.X48, readlane<48>(cb), cb);
+        // two options per wave: each takes the first lane of its own second row
+        const T src = t.wrow < 2 ? readlane<16>(cb) : readlane<48>(cb);
+        cb = fma(sm.X16, src, cb);
     }
     T zin;
     if constexpr (W == 2) {
@@ -464,7 +467,9 @@ __device__ __forceinline__ void march_f64(double (&xv)[M], const double (&ap)[M]
         for (int i = M - 2; i >= 0; --i) g[i] = fma(-up[i], g[i + 1], g[i]);
         const double gin = fwd_carry<L, W>(c, sm, t, first_lane, xchg);
         const double win = bwd_carry<L, W>(fma(R[0], gin, g[0]), sm, t, last_lane, xchg);
-        if (step < steps) {
+        // one option per workgroup: every step applies (no branch, so the LDS reads and
+        // fix-ups can be scheduled into the backward scan)
+        if (Geo<M, L, W>::G == 1 || step < steps) {
 #pragma unroll
             for (int i = 0; i < M; ++i) {
                 const double2 qv = qpv[i * NT + t.gl];
@@ -591,7 +596,7 @@ __device__ __forceinline__ void solve_group_mixed(const FdBatch& b, int group, d
         for (int i = M - 2; i >= 0; --i) g[i] = fmaf(-up[i], g[i + 1], g[i]);
         const float gin = fwd_carry<L, W>(c, sm, t, first_lane, xf);
         const float win = bwd_carry<L, W>(fmaf(R[0], gin, g[0]), sm, t, last_lane, xf);
-        if (step < d.steps) {
+        if (Geo<M, L, W>::G == 1 || step < d.steps) {
 #pragma unroll
             for (int i = 0; i < M; ++i) {
                 float w = fmaf(R[i], gin, g[i]);

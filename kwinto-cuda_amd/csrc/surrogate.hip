@@ -42,7 +42,8 @@ constexpr uint32_t kFileVersion = 2;  // saved parameters belong to this target
 
 // Barone-Adesi & Whaley (1987) quadratic approximation of the early-exercise premium
 // (American minus European Black-Scholes) with a continuous dividend yield q. With
-// M = 2r / sigma^2, N = 2 (r - q) / sigma^2, k(T) = 1 - e^{-rT},
+// M = 2r / sigma^2, N = 2 (r - q) / sigma^2, k(T) = 1 - e^{-rT} (M / k -> 2 / (sigma^2 T)
+// as r -> 0),
 // q_w = (-(N - 1) + w sqrt((N - 1)^2 + 4 M / k(T))) / 2, the critical spot S* solves
 //     w (S* - K) = E(S*) + w (1 - e^{-qT} Phi(w d1(S*))) S* / q_w,
 // and the premium is A (S / S*)^{q_w} with A = w (S* / q_w)(1 - e^{-qT} Phi(w d1(S*)))
@@ -56,7 +57,8 @@ __host__ __device__ inline double baw_premium(const kwgpu_option& o) {
     if (!(s > 0 && k > 0 && t > 0 && z > 0)) return 0.0;
     if ((w > 0 && !(q > 0)) || (w < 0 && !(r > 0))) return 0.0;
     const double z2 = z * z;
-    const double mk = 2 * r / (z2 * -expm1(-r * t));
+    // M / k(T), with its limit 2 / (sigma^2 T) at r = 0 (calls with q > 0 exercise early)
+    const double mk = r > 0 ? 2 * r / (z2 * -expm1(-r * t)) : 2 / (z2 * t);
     const double nn = 2 * (r - q) / z2;
     const double qw = 0.5 * (-(nn - 1) + w * sqrt((nn - 1) * (nn - 1) + 4 * mk));
     const double sq = z * sqrt(t), eq = exp(-q * t), er = exp(-r * t);

@@ -35,14 +35,18 @@ def to_device(opts):
     return torch.from_numpy(np.ascontiguousarray(opts).view(np.uint8).copy()).cuda()
 
 
-def fd_prices(ctx: Context, d_opts, n: int, precision=FP64, x_dim=X_DIM, t_dim=T_DIM):
-    """FD prices of device options (device float64 tensor); raises on any failure."""
+def fd_prices(ctx: Context, d_opts, n: int, precision=FP64, x_dim=X_DIM, t_dim=T_DIM,
+              with_status=False):
+    """FD prices of device options (device float64 tensor). Raises on any failed option
+    unless `with_status`, which returns (prices, status) instead."""
     torch = _torch()
     d_p = torch.empty(n, dtype=torch.float64, device="cuda")
     d_s = torch.empty(n, dtype=torch.int32, device="cuda")
     ctx.fd1d_price_device(d_opts.data_ptr(), n, d_p.data_ptr(), d_s.data_ptr(),
                           params(x_dim, t_dim, precision=precision))
     ctx.synchronize()
+    if with_status:
+        return d_p, d_s
     bad = int((d_s != 0).sum().item())
     if bad:
         raise RuntimeError(f"{bad} FD solves failed")
@@ -62,17 +66,27 @@ def bs_euro(ctx: Context, opts) -> "object":
 
 
 def premium_labels(ctx: Context, opts, chunk=1 << 18):
-    """(device options, device premium = FD FP64 - BS) for a host option array."""
+    """(device options, device premium = FD FP64 - BS, kept) for a host option array.
+    Options the FD pricer rejects (a spot outside its grid: the box's corner K = 200,
+    sigma = 0.1, T = 1/52 has log(S/K) at the grid's end) are dropped; `kept` is the
+    host index of the options returned."""
     torch = _torch()
     n = len(opts)
     d_o = to_device(opts)
     prem = torch.empty(n, dtype=torch.float64, device="cuda")
+    st = torch.empty(n, dtype=torch.int32, device="cuda")
     bs = bs_euro(ctx, opts)
     for c0 in range(0, n, chunk):
         c1 = min(n, c0 + chunk)
-        prem[c0:c1] = fd_prices(ctx, d_o[c0 * 56:c1 * 56], c1 - c0)
+        prem[c0:c1], st[c0:c1] = fd_prices(ctx, d_o[c0 * 56:c1 * 56], c1 - c0, with_status=True)
     prem -= bs
-    return d_o, prem
+    ok = (st == 0).cpu().numpy()
+    kept = np.nonzero(ok)[0]
+    if len(kept) < n:
+        sel = torch.from_numpy(kept).cuda()
+        d_o = d_o.view(n, 56)[sel].reshape(-1).contiguous()
+        prem = prem[sel].contiguous()
+    return d_o, prem, kept
 
 
 def predict(ctx: Context, sur: Surrogate, d_opts, n: int):
@@ -100,9 +114,13 @@ def train_and_evaluate(ctx: Context, n_train=2_000_000, n_test=65536, members=4,
     """Trains the ensemble on FD labels and evaluates it on held-out FD labels."""
     t0 = time.perf_counter()
     opts = W.surrogate_train(n_train + n_test, seed)
-    d_o, prem = premium_labels(ctx, opts)
+    d_o, prem, kept = premium_labels(ctx, opts)
     t_label = time.perf_counter() - t0
-    log(f"labels: {n_train + n_test} options by FD FP64 {X_DIM}x{T_DIM} in {t_label:.2f} s")
+    dropped = n_train + n_test - len(kept)
+    n_train = int(np.sum(kept < n_train))  # kept options keep their order
+    n_test = len(kept) - n_train
+    log(f"labels: {len(kept)} options by FD FP64 {X_DIM}x{T_DIM} in {t_label:.2f} s "
+        f"({dropped} outside their grid dropped)")
     sur = Surrogate(ctx, members=members, seed=seed)
     t0 = time.perf_counter()
     loss = sur.train(d_o.data_ptr(), prem.data_ptr(), n_train, epochs=epochs, batch=batch)
@@ -117,7 +135,8 @@ def train_and_evaluate(ctx: Context, n_train=2_000_000, n_test=65536, members=4,
     thr = calibrate_spread(err, spread)
     served = spread <= thr
     out = {
-        "n_train": n_train, "n_test": n_test, "members": members, "epochs": epochs,
+        "n_train": n_train, "n_test": n_test, "dropped_outside_grid": dropped,
+        "members": members, "epochs": epochs,
         "label_s": t_label, "train_s": t_train, "final_loss": loss[-1].tolist(),
         "heldout_max_abs_err": float(np.max(np.abs(err))),
         "heldout_rms_abs_err": float(np.sqrt(np.mean(err * err))),

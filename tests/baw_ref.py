@@ -24,7 +24,7 @@ def critical_spot(o):
     w = np.where(o["w"] < 0, -1.0, 1.0)
     z2 = z * z
     with np.errstate(divide="ignore", invalid="ignore"):
-        mk = 2 * r / (z2 * -np.expm1(-r * t))
+        mk = np.where(r > 0, 2 * r / (z2 * -np.expm1(-r * t)), 2 / (z2 * t))
     nn = 2 * (r - q) / z2
     qw = 0.5 * (-(nn - 1) + w * np.sqrt((nn - 1) ** 2 + 4 * mk))
     eq = np.exp(-q * t)

@@ -198,6 +198,28 @@ def test_bad_inputs_and_params(ctx):
     assert rc == kwgpu.OK
 
 
+@pytest.mark.parametrize("x_dim", [100, 300, 500, 1000, 2000])
+def test_neighbours_of_bad_options_unaffected(ctx, x_dim):
+    """Options sharing a wave with an invalid option, with one whose march overflows
+    (its far call payoff is inf) and with an empty segment (odd count) are priced as if
+    alone: nothing crosses from one option's lanes into another's."""
+    for n in (7, 8):  # the longest-dated (overflowing) option last: with 8 it shares a
+        _say("neighbours", x_dim, n)  # wave with a valid option, with 7 with nothing
+        o = W.plain(n, 61)
+        o["s"] = 100
+        o["z"][1] = -0.2                                      # invalid
+        o["w"][-1], o["t"][-1], o["z"][-1] = 1, 30.0, 3.0     # e^(50 sigma sqrt T) = inf
+        ref, _ = oracle.fd1d_price_status(o, x_dim, 20)
+        good = np.ones(n, bool)
+        good[[1, n - 1]] = False
+        for lay in LAYOUTS:
+            p, st, rc, err = ctx.fd1d_price(o, x_dim, 20, layout=lay)
+            assert st[1] == kwgpu.STATUS_BAD_INPUT
+            assert st[-1] == kwgpu.STATUS_NONFINITE or not np.isfinite(p[-1])
+            assert np.all(st[good] == 0)
+            assert np.max(np.abs(p[good] - ref[good])) <= EPS, lay
+
+
 def test_chain_compression(ctx):
     """Options sharing (t, r, q, z, e, w) solve one PDE and interpolate per spot/strike
     (src/Pricer/kwFd1d.cpp:28-65, :146-157)."""

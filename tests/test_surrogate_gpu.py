@@ -148,13 +148,14 @@ def test_baseline_matches_numpy_restatement(ctx):
     the no-early-exercise cases (puts at r = 0, calls at q = 0)."""
     from baw_ref import baw_premium
     o = W.surrogate_train(4096, 12)
-    o[:64]["r"] = 0.0
-    o[64:128]["q"] = 0.0
+    o["r"][:64] = 0.0     # puts: no premium; calls with q > 0: the r -> 0 limit
+    o["q"][64:128] = 0.0  # calls: no premium
     got = kwgpu.surrogate_baseline(ctx, o)
     ref = baw_premium(o)
     assert np.all(np.isfinite(got))
     assert np.max(np.abs(got - ref) / o["k"]) < 1e-9
-    assert np.all(got[:64][o[:64]["w"] < 0] == 0) and np.all(got[64:128][o[64:128]["w"] > 0] == 0)
+    assert np.all(got[:64][o["w"][:64] < 0] == 0) and np.all(got[64:128][o["w"][64:128] > 0] == 0)
+    assert np.all(got[:64][(o["w"][:64] > 0) & (o["q"][:64] > 0)] >= 0)
 
 
 def _fd_premium(ctx, opts, x_dim, t_dim, precision=kwgpu.FP64):

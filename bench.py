@@ -165,10 +165,13 @@ def bench_surrogate(ctx, args, log):
     from kwgpu import workloads as W
     sur, train = pipeline.train_and_evaluate(ctx, n_train=args.sur_train,
                                              epochs=args.sur_epochs, log=log)
-    serve = pipeline.serve(ctx, sur, W.serving(args.sur_serve, 4), train["spread_bound"],
-                           log=log)
+    opts = W.serving(args.sur_serve, 4)
+    serve = pipeline.serve(ctx, sur, opts, train["spread_bound"], log=log)
+    # the trade-off at looser bars (not the metric: its bar is 1e-4)
+    looser = {bar: pipeline.serve(ctx, sur, opts, v["spread_bound"], repeats=1, log=log)
+              for bar, v in train["looser_bars"].items()}
     sur.close()
-    return {"training": train, "serving": serve}
+    return {"training": train, "serving": serve, "serving_at_looser_bars": looser}
 
 
 def main():

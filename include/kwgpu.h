@@ -194,11 +194,11 @@ void* kwgpu_stream(kwgpu_ctx* ctx);
  * of the FD pricer above and uses it as its source of labels and as its fallback).
  *
  * An ensemble of MLPs 6 -> 256 -> 256 -> 256 -> 256 -> 1 with SiLU predicts
- * y = 100 (premium - baseline) / K, where premium = FD American price - Black-Scholes
- * European price and baseline is the Barone-Adesi & Whaley quadratic approximation of
- * that premium (kwgpu_surrogate_baseline; a control variate that leaves the network
- * about a tenth of the premium to learn), from features (log(S/K), T, sigma, r, q, w)
- * normalised to the box
+ * y = (premium - baseline) / (baseline + 1e-4 K), where premium = FD American price -
+ * Black-Scholes European price and baseline is the Barone-Adesi & Whaley quadratic
+ * approximation of that premium (kwgpu_surrogate_baseline; a control variate that leaves
+ * the network a fraction of the premium to learn, and whose size sets the price scale of
+ * the network's error), from features (log(S/K), T, sigma, r, q, w) normalised to the box
  * K/S in [0.4, 2], T in [1/52, 5], sigma in [0.1, 0.6], r in [0, 0.1], q in [0, 0.08].
  * The three 256x256 layers run on BF16 MFMA with FP32 accumulation (forward and
  * backward); the 6-wide input layer and the 1-wide output layer run in FP32; Adam

@@ -170,90 +170,188 @@ __device__ __forceinline__ Pde pde_of(const FdBatch& b, int group, const Lanes& 
 }
 
 // ---- set-up: grid, operator rows, payoff, LU factors, carry coefficients (FP64) ----
+// Reciprocal from v_rcp_f64 and two Newton steps (within an ulp or two of 1/x for the
+// normal, finite x used here).
+__device__ __forceinline__ double rcp(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(r, fma(-x, r, 1.0), r);
+    return fma(r, fma(-x, r, 1.0), r);
+}
+
+// The sinh mesh x_j = density sinh(y_j), y_j = y_min (1 - j dy) + y_max j dy
+// (src/Pricer/kwFd1d.cpp:105-125), for K consecutive nodes from j: sinh as
+// (e^y - e^-y) / 2 with e^y_j a geometric progression (one exp for the first node, one
+// for the ratio). Its error, a few 1e-16 relative in e^y, moves no price by more than
+// ~1e-13.
+template <int K>
+__device__ __forceinline__ void mesh_run(const PdeParams& p, double density, int j,
+                                         double (&x)[K]) {
+#pragma clang fp contract(off)
+    const double yj = j * p.dy;
+    double e = exp(p.yMin * (1.0 - yj) + p.yMax * yj);
+    double f = rcp(e);
+    const double step = (p.yMax - p.yMin) * p.dy;
+    const double up = exp(step), down = rcp(up);
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+        x[i] = density * (0.5 * (e - f));
+        e *= up;
+        f *= down;
+    }
+}
+
+// Row j of B (src/Math/kwFd1d.cpp:73-114) from the reciprocals of x_j - x_{j-1} (rl),
+// x_{j+1} - x_j (ru) and x_{j+1} - x_{j-1} (rm); rows j >= N are identity padding.
+__device__ __forceinline__ void b_row_rcp(const PdeParams& p, double th, int j, double rl,
+                                          double ru, double rm, double& bl, double& b,
+                                          double& bu) {
+#pragma clang fp contract(off)
+    const int N = p.x_dim;
+    const double dt = p.dt, a0 = p.a0, ax = p.ax, axx = p.axx;
+    if (j >= N) {
+        bl = 0.0;
+        b = 1.0;
+        bu = 0.0;
+    } else if (j == 0) {
+        bl = 0;
+        b = 1 - th * dt * (a0 - ru * ax);
+        bu = -th * dt * (ru * ax);
+    } else if (j == N - 1) {
+        bl = -th * dt * (-rl * ax);
+        b = 1 - th * dt * (a0 + rl * ax);
+        bu = 0;
+    } else {
+        const double inv_dx2u = 2. * ru * rm;
+        const double inv_dx2m = 2. * rl * ru;
+        const double inv_dx2l = 2. * rl * rm;
+        bl = -th * dt * (-rm * ax + inv_dx2l * axx);
+        b = 1 - th * dt * (a0 - inv_dx2m * axx);
+        bu = -th * dt * (rm * ax + inv_dx2u * axx);
+    }
+}
+
+// 2x2 projective map [[a, b], [c, d]] of the pivot recurrence.
+struct Map2 {
+    double a, b, c, d;
+};
+
+// x . y, rescaled by a power of two (exact) so that its largest entry lies in [0.5, 1):
+// the pivot a map yields, (a + b) / (c + d), does not depend on the scale.
+__device__ __forceinline__ Map2 map_mul(const Map2& x, const Map2& y) {
+    Map2 m{x.a * y.a + x.b * y.c, x.a * y.b + x.b * y.d, x.c * y.a + x.d * y.c,
+           x.c * y.b + x.d * y.d};
+    const double s = fmax(fmax(fabs(m.a), fabs(m.b)), fmax(fabs(m.c), fabs(m.d)));
+    const int e = (s > 0.0 && s < INFINITY) ? -__builtin_amdgcn_frexp_exp(s) : 0;
+    return {ldexp(m.a, e), ldexp(m.b, e), ldexp(m.c, e), ldexp(m.d, e)};
+}
+
+template <int CTRL, int ROWMASK = 0xF>
+__device__ __forceinline__ Map2 dpp_map(const Map2& m) {
+    return {dpp<CTRL, ROWMASK>(m.a), dpp<CTRL, ROWMASK>(m.b), dpp<CTRL, ROWMASK>(m.c),
+            dpp<CTRL, ROWMASK>(m.d)};
+}
+
+__device__ __forceinline__ Map2 select_map(bool p, const Map2& x, const Map2& y) {
+    return {p ? x.a : y.a, p ? x.b : y.b, p ? x.c : y.c, p ? x.d : y.d};
+}
+
 // qp: LDS scratch of at least 4 * NT doubles. On return xv holds the payoff, ap the
 // scaled inverse pivots 1/(theta d), lo/up the unit-lower and scaled upper factors
 // l_j = bl_j / d_{j-1}, u_j = bu_j / d_{j+1}, R/Q the carry fix-ups, blv/buv the
-// off-diagonals of B, and sm the scan multipliers.
+// off-diagonals of B, and sm the scan multipliers. Every cross-lane step below reads
+// only lanes of the same option, or discards what it read with a select.
 template <int M, int L, int W>
 __device__ __forceinline__ void setup(const FdBatch& b, const Pde& d, const Lanes& t, double* qp,
                                       double (&xv)[M], double (&ap)[M], double (&lo)[M],
                                       double (&up)[M], double (&R)[M], double (&Q)[M],
                                       double (&blv)[M], double (&buv)[M], ScanMul<double>& sm) {
     using Gm = Geo<M, L, W>;
-    constexpr int NT = Gm::NT, LT = Gm::LT;
+    constexpr int LT = Gm::LT;
     const PdeParams& P = d.P;
     const int N = d.N, j0 = d.j0, gl = t.gl;
+    const int pr = t.pr, wrow = t.wrow, wv = t.wv;
     const double th = b.theta;
     const double inv_th = 1.0 / th;
     double bv[M];
     double bu_prev = 0.0;  // bu of row j0-1
     {
         double xg[M + 3];  // x_{j0-2} .. x_{j0+M}
+        mesh_run(P, b.density, j0 - 2, xg);
 #pragma unroll
         for (int i = 0; i < M + 3; ++i) {
             const int j = j0 - 2 + i;
-            xg[i] = (j >= 0 && j < N) ? grid_x(P, b.density, j) : 0.0;
+            // nodes outside the mesh only feed padding rows; keep their spacings nonzero
+            if (j < 0 || j >= N) xg[i] = (j < 0 ? -1e6 : 1e6) + j;
         }
+        double rdx[M + 2], rdm[M + 1];  // 1/(x_{i+1} - x_i), 1/(x_{i+2} - x_i) over xg
+#pragma unroll
+        for (int i = 0; i < M + 2; ++i) rdx[i] = rcp(xg[i + 1] - xg[i]);
+#pragma unroll
+        for (int i = 0; i < M + 1; ++i) rdm[i] = rcp(xg[i + 2] - xg[i]);
         if (j0 >= 1 && j0 - 1 < N) {
             double t0, t1;
-            b_row(P, th, j0 - 1, xg[0], xg[1], xg[2], t0, t1, bu_prev);
+            b_row_rcp(P, th, j0 - 1, rdx[0], rdx[1], rdm[0], t0, t1, bu_prev);
         }
 #pragma unroll
         for (int i = 0; i < M; ++i) {
             const int j = j0 + i;  // x_{j-1}, x_j, x_{j+1} = xg[i+1], xg[i+2], xg[i+3]
-            b_row(P, th, j, xg[i + 1], xg[i + 2], xg[i + 3], blv[i], bv[i], buv[i]);
+            b_row_rcp(P, th, j, rdx[i + 1], rdx[i + 2], rdm[i + 1], blv[i], bv[i], buv[i]);
             xv[i] = j < N ? payoff(P, xg[i + 2]) : 0.0;
         }
     }
 
     // LU pivots d_j = b_j - bl_j bu_{j-1} / d_{j-1} are a serial recurrence; its value
-    // entering each lane comes from a prefix product of the lanes' projective maps
-    // (u, v) -> (b u - bl bu_prev v, u), normalised to avoid overflow.
-    double m00 = 1, m01 = 0, m10 = 0, m11 = 1;
+    // entering each lane is the projective image of the prefix product of the lanes'
+    // maps (u, v) -> (b u - bl bu_prev v, u), taken by a Kogge-Stone scan over lanes.
+    Map2 m{1, 0, 0, 1};
     {
         double bup = bu_prev;
 #pragma unroll
         for (int i = 0; i < M; ++i) {
             const double e = blv[i] * bup;
-            const double n00 = bv[i] * m00 - e * m10, n01 = bv[i] * m01 - e * m11;
-            m10 = m00;
-            m11 = m01;
-            m00 = n00;
-            m01 = n01;
-            const double s = fmax(fmax(fabs(m00), fabs(m01)), fmax(fabs(m10), fabs(m11)));
-            const double is = s > 0.0 ? 1.0 / s : 1.0;
-            m00 *= is; m01 *= is; m10 *= is; m11 *= is;
+            m = map_mul(Map2{bv[i], -e, 1.0, 0.0}, m);
             bup = buv[i];
         }
     }
-    __syncthreads();  // the previous group's interpolation may still read qp
-    qp[4 * gl + 0] = m00;
-    qp[4 * gl + 1] = m01;
-    qp[4 * gl + 2] = m10;
-    qp[4 * gl + 3] = m11;
-    __syncthreads();
-    double s00 = 1, s01 = 0, s10 = 0, s11 = 1;
-    for (int k = t.seg * LT; k < gl; ++k) {
-        const double a00 = qp[4 * k], a01 = qp[4 * k + 1], a10 = qp[4 * k + 2], a11 = qp[4 * k + 3];
-        const double n00 = a00 * s00 + a01 * s10, n01 = a00 * s01 + a01 * s11;
-        const double n10 = a10 * s00 + a11 * s10, n11 = a10 * s01 + a11 * s11;
-        const double s = fmax(fmax(fabs(n00), fabs(n01)), fmax(fabs(n10), fabs(n11)));
-        const double is = s > 0.0 ? 1.0 / s : 1.0;
-        s00 = n00 * is; s01 = n01 * is; s10 = n10 * is; s11 = n11 * is;
+    // inclusive prefix within the row, then across rows of the option
+    Map2 s = m;
+    s = select_map(pr >= 1, map_mul(s, dpp_map<kRowShr + 1>(s)), s);
+    s = select_map(pr >= 2, map_mul(s, dpp_map<kRowShr + 2>(s)), s);
+    s = select_map(pr >= 4, map_mul(s, dpp_map<kRowShr + 4>(s)), s);
+    s = select_map(pr >= 8, map_mul(s, dpp_map<kRowShr + 8>(s)), s);
+    if constexpr (L >= 32) s = select_map(wrow & 1, map_mul(s, dpp_map<kRowBcast15, 0xA>(s)), s);
+    if constexpr (L == 64) s = select_map(wrow >= 2, map_mul(s, dpp_map<kRowBcast31, 0xC>(s)), s);
+    // the map entering this lane: the previous lane's inclusive product
+    Map2 e = dpp_map<kWaveShr1>(s);
+    if constexpr (W == 2) {
+        __syncthreads();  // the previous group's interpolation may still read qp
+        if (wv == 0 && t.lane == 63) {
+            qp[0] = s.a;
+            qp[1] = s.b;
+            qp[2] = s.c;
+            qp[3] = s.d;
+        }
+        __syncthreads();
+        if (wv == 1) {
+            const Map2 w0{qp[0], qp[1], qp[2], qp[3]};
+            e = t.lane == 0 ? w0 : map_mul(e, w0);
+        }
     }
-    double dv[M];
+    double dv[M], rdv[M];
     {
-        double dprev = t.ls == 0 ? 1.0 : (s00 + s01) / (s10 + s11);
-        // Thomas pivots of the reference (gam_j = bu_{j-1}/bet, bet = b_j - bl_j gam_j)
+        double dprev = t.ls == 0 ? 1.0 : (e.a + e.b) / (e.c + e.d);
         double bup = bu_prev;
+        double rprev = rcp(dprev);
 #pragma unroll
         for (int i = 0; i < M; ++i) {
-            const double gam = bup / dprev;
-            const double dd = bv[i] - blv[i] * gam;
-            lo[i] = blv[i] / dprev;  // bl = 0 on row 0, whatever the entering pivot
+            // Thomas pivots of the reference (gam_j = bu_{j-1}/bet, bet = b_j - bl_j gam_j)
+            const double dd = bv[i] - blv[i] * (bup * rprev);
+            const double rd = rcp(dd);
+            lo[i] = blv[i] * rprev;  // bl = 0 on row 0, whatever the entering pivot
             dv[i] = dd;
-            ap[i] = inv_th / dd;
-            dprev = dd;
+            rdv[i] = rd;
+            ap[i] = inv_th * rd;
+            rprev = rd;
             bup = buv[i];
         }
     }
@@ -263,9 +361,9 @@ __device__ __forceinline__ void setup(const FdBatch& b, const Pde& d, const Lane
     qp[gl] = dv[0];
     __syncthreads();
     {
-        const double dnext = t.ls + 1 < LT ? qp[gl + 1] : 1.0;
+        const double rnext = t.ls + 1 < LT ? rcp(qp[gl + 1]) : 1.0;
 #pragma unroll
-        for (int i = 0; i < M; ++i) up[i] = buv[i] / (i + 1 < M ? dv[i + 1] : dnext);
+        for (int i = 0; i < M; ++i) up[i] = buv[i] * (i + 1 < M ? rdv[i + 1] : rnext);
     }
 
     // Carry propagation coefficients: g_j = ghat_j + Pc_j g_in,
@@ -285,35 +383,58 @@ __device__ __forceinline__ void setup(const FdBatch& b, const Pde& d, const Lane
         Pl = Pc[M - 1];
         Ql = Q[0];
     }
-    __syncthreads();
-    qp[gl] = Pl;
-    qp[NT + gl] = Ql;
-    __syncthreads();
-    const int pr = t.pr, row16 = t.row16, wrow = t.wrow, wv = t.wv;
-    sm.F1 = pr >= 1 ? lds_prod(qp, gl, gl) : 0.0;
-    sm.F2 = pr >= 2 ? lds_prod(qp, gl - 1, gl) : 0.0;
-    sm.F4 = pr >= 4 ? lds_prod(qp, gl - 3, gl) : 0.0;
-    sm.F8 = pr >= 8 ? lds_prod(qp, gl - 7, gl) : 0.0;
-    sm.FB15 = (L >= 32 && (wrow & 1)) ? lds_prod(qp, 16 * row16, gl) : 0.0;
-    sm.FB31 = (L == 64 && wrow >= 2) ? lds_prod(qp, 64 * wv + 32, gl) : 0.0;
-    const double* lq = qp + NT;
-    sm.G1 = pr + 1 <= 15 ? lds_prod(lq, gl, gl) : 0.0;
-    sm.G2 = pr + 2 <= 15 ? lds_prod(lq, gl, gl + 1) : 0.0;
-    sm.G4 = pr + 4 <= 15 ? lds_prod(lq, gl, gl + 3) : 0.0;
-    sm.G8 = pr + 8 <= 15 ? lds_prod(lq, gl, gl + 7) : 0.0;
+    // Scan multipliers: products of the lanes' Pl (forward) and Ql (backward) over the
+    // windows each scan stage spans. Lanes whose window leaves the row take 0 by select.
     {
-        // product of Q from this lane to the end of its row, applied to the first lane of
-        // the next row; each readlane stage updates one row only
-        const double xb = lds_prod(lq, gl, 16 * row16 + 15);
+        const double w2 = Pl * dpp<kRowShr + 1>(Pl);  // Pl over lanes l-1 .. l
+        const double w4 = w2 * dpp<kRowShr + 2>(w2);  // l-3 .. l
+        const double w8 = w4 * dpp<kRowShr + 4>(w4);  // l-7 .. l
+        sm.F1 = pr >= 1 ? Pl : 0.0;
+        sm.F2 = pr >= 2 ? w2 : 0.0;
+        sm.F4 = pr >= 4 ? w4 : 0.0;
+        sm.F8 = pr >= 8 ? w8 : 0.0;
+        const double v2 = Ql * dpp<kRowShl + 1>(Ql);  // Ql over lanes l .. l+1
+        const double v4 = v2 * dpp<kRowShl + 2>(v2);  // l .. l+3
+        const double v8 = v4 * dpp<kRowShl + 4>(v4);  // l .. l+7
+        sm.G1 = pr + 1 <= 15 ? Ql : 0.0;
+        sm.G2 = pr + 2 <= 15 ? v2 : 0.0;
+        sm.G4 = pr + 4 <= 15 ? v4 : 0.0;
+        sm.G8 = pr + 8 <= 15 ? v8 : 0.0;
+        // row-inclusive prefix of Pl and suffix of Ql
+        double ip = Pl, sq = Ql;
+        {
+            double x;
+            x = dpp<kRowShr + 1>(ip); ip = pr >= 1 ? ip * x : ip;
+            x = dpp<kRowShr + 2>(ip); ip = pr >= 2 ? ip * x : ip;
+            x = dpp<kRowShr + 4>(ip); ip = pr >= 4 ? ip * x : ip;
+            x = dpp<kRowShr + 8>(ip); ip = pr >= 8 ? ip * x : ip;
+            x = dpp<kRowShl + 1>(sq); sq = pr + 1 <= 15 ? sq * x : sq;
+            x = dpp<kRowShl + 2>(sq); sq = pr + 2 <= 15 ? sq * x : sq;
+            x = dpp<kRowShl + 4>(sq); sq = pr + 4 <= 15 ? sq * x : sq;
+            x = dpp<kRowShl + 8>(sq); sq = pr + 8 <= 15 ? sq * x : sq;
+        }
+        // row products (uniform): Pl over rows 0..2 from the prefix at each row's end,
+        // Ql over rows 1..3 from the suffix at each row's start
+        const double rp0 = readlane<15>(ip), rp1 = readlane<31>(ip), rp2 = readlane<47>(ip);
+        const double rs1 = readlane<16>(sq), rs2 = readlane<32>(sq), rs3 = readlane<48>(sq);
+        sm.FB15 = (L >= 32 && (wrow & 1)) ? ip : 0.0;
+        sm.FB31 = (L == 64 && wrow >= 2) ? (wrow == 3 ? ip * rp2 : ip) : 0.0;
         const bool w64 = L == 64, w32 = L == 32;
-        sm.X16 = (w64 && wrow == 0) || (w32 && (wrow & 1) == 0) ? xb : 0.0;
-        sm.X32 = w64 && wrow == 1 ? xb : 0.0;
-        sm.X48 = w64 && wrow == 2 ? xb : 0.0;
+        sm.X16 = (w64 && wrow == 0) || (w32 && (wrow & 1) == 0) ? sq : 0.0;
+        sm.X32 = w64 && wrow == 1 ? sq : 0.0;
+        sm.X48 = w64 && wrow == 2 ? sq : 0.0;
+        // two-wave groups: Pl over wave 1's lanes 0..l (carry of wave 0's last lane into
+        // wave 1) and Ql over wave 0's lanes l..63 (carry of wave 1's first lane back)
+        if constexpr (W == 2) {
+            const double before = wrow == 0 ? 1.0 : wrow == 1 ? rp0 : wrow == 2 ? rp0 * rp1 : rp0 * rp1 * rp2;
+            const double after = wrow == 3 ? 1.0 : wrow == 2 ? rs3 : wrow == 1 ? rs2 * rs3 : rs1 * rs2 * rs3;
+            sm.FW = wv == 1 ? ip * before : 0.0;
+            sm.ZW = wv == 0 ? sq * after : 0.0;
+        } else {
+            sm.FW = 0.0;
+            sm.ZW = 0.0;
+        }
     }
-    // two-wave groups: carry of wave 0's last lane into wave 1, and of wave 1's first
-    // lane back into wave 0
-    sm.FW = (W == 2 && wv == 1) ? lds_prod(qp, 64, gl) : 0.0;
-    sm.ZW = (W == 2 && wv == 0) ? lds_prod(lq, gl, 63) : 0.0;
 }
 
 // All segments of a workgroup iterate to its longest march; lanes whose own march is
@@ -495,7 +616,8 @@ __device__ __forceinline__ void solve_group_f64(const FdBatch& b, int group, dou
     setup<M, L, W>(b, d, t, qp, xv, ap, lo, up, R, Q, blv, buv, sm);
     const bool first_lane = t.ls == 0;
     const bool last_lane = t.ls == Gm::LT - 1;
-    const int wsteps = group_steps<M, L, W>(d, t, qp);
+    // the same for every lane of the workgroup: keep the loop counter scalar
+    const int wsteps = __builtin_amdgcn_readfirstlane(group_steps<M, L, W>(d, t, qp));
 
     // (Q_i, projection target_i) pairs of every thread at qpv[i * NT + gl]
     __syncthreads();
@@ -550,7 +672,8 @@ __device__ __forceinline__ void solve_group_mixed(const FdBatch& b, int group, d
     const double sdiag = b.theta * d.P.dt * d.P.a0;
     const bool first_lane = t.ls == 0;
     const bool last_lane = t.ls == Gm::LT - 1;
-    const int wsteps = group_steps<M, L, W>(d, t, qp);
+    // the same for every lane of the workgroup: keep the loop counter scalar
+    const int wsteps = __builtin_amdgcn_readfirstlane(group_steps<M, L, W>(d, t, qp));
     float* xf = reinterpret_cast<float*>(xchg + 2);  // carries; xchg[0..1] exchange V
 
     // projection targets of every thread at qp[i * NT + gl]

@@ -35,10 +35,12 @@ constexpr int kW1 = 0, kB1 = kW1 + H * F, kW2 = kB1 + H, kB2 = kW2 + H * H, kW3 
               kB3 = kW3 + H * H, kW4 = kB3 + H, kB4 = kW4 + H * H, kW5 = kB4 + H, kB5 = kW5 + H,
               kP = kB5 + 1;
 static_assert(kP == KWGPU_SURROGATE_PARAMS, "parameter layout");
-// Target: y = kYScale (premium - baw_premium) / K, the part of the early-exercise premium
-// that the quadratic approximation below misses, per unit strike (about 0.3 rms).
-constexpr double kYScale = 100.0;
-constexpr uint32_t kFileVersion = 2;  // saved parameters belong to this target
+// Target: y = (premium - baw) / (baw + kEps K), the part of the early-exercise premium
+// that the quadratic approximation below (baw) misses, relative to that approximation
+// (floored at kEps K). A network error dy then costs (baw + kEps K) dy in price: small
+// where the premium is small, which is where an absolute error bar can be met.
+constexpr double kEps = 1e-4;
+constexpr uint32_t kFileVersion = 3;  // saved parameters belong to this target
 
 // Barone-Adesi & Whaley (1987) quadratic approximation of the early-exercise premium
 // (American minus European Black-Scholes) with a continuous dividend yield q. With
@@ -87,7 +89,7 @@ __host__ __device__ inline double baw_premium(const kwgpu_option& o) {
 }
 
 __host__ __device__ inline double premium_of(double y, double k, double baw) {
-    return baw + k * y / kYScale;
+    return baw + (baw + kEps * k) * y;
 }
 
 // Training box, as (centre, half-width) of each raw feature.
@@ -128,7 +130,10 @@ __global__ void features_kernel(const kwgpu_option* opts, size_t n, const double
     float f[F];
     const bool in = option_features(o, f);
     for (int k = 0; k < F; ++k) feat[i * F + k] = f[k];
-    if (target) target[i] = (float)((premium[i] - baw_premium(o)) * kYScale / o.k);
+    if (target) {
+        const double baw = baw_premium(o);
+        target[i] = (float)((premium[i] - baw) / (baw + kEps * o.k));
+    }
     if (in_box) in_box[i] = in ? 1 : 0;
 }
 

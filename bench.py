@@ -57,6 +57,9 @@ def parse_args():
     ap.add_argument("--sur-train", type=int, default=2_000_000)
     ap.add_argument("--sur-serve", type=int, default=1 << 20)
     ap.add_argument("--sur-epochs", type=int, default=30)
+    ap.add_argument("--peak-probe", type=int, default=1,
+                    help="run bin/kwgpu_peak after the timed work (measured FP64 FMA rate "
+                         "and copy bandwidth)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="after the timed region, run one device copy of this size "
                          "(calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -82,6 +85,24 @@ def error_by_class(opts, d):
             m = (opts["w"] == w) & (opts["t"] >= lo) & (opts["t"] < hi)
             out[f"{wn} {tn}"] = float(d[m].max()) if m.any() else None
     return out
+
+
+def device_ceilings(achieved_tflops):
+    """Measured FP64 FMA throughput and copy bandwidth of this device (bin/kwgpu_peak,
+    run as a child process after all timed work), beside the published figures."""
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "kwinto-cuda_amd", "bin",
+                       "kwgpu_peak")
+    if not os.path.exists(exe):
+        return {"error": "kwgpu_peak not built"}
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    if r.returncode != 0:
+        return {"error": r.stderr.strip()[-300:]}
+    m = json.loads(r.stdout.strip().splitlines()[-1])
+    m["spec_fp64_tflops"] = FP64_PEAK_TFLOPS
+    m["spec_hbm_gb_per_s"] = 8000.0
+    m["fd_fp64_frac_of_measured_fma"] = achieved_tflops / m["fp64_fma_tflops"]
+    return m
 
 
 def cpu_model():
@@ -267,6 +288,9 @@ def main():
             "options_per_s": extra["surrogate"]["serving"]["surrogate_first_options_per_s"],
             "max_abs_err_vs_fp64_fd": extra["surrogate"]["serving"]["max_abs_err_vs_fp64_fd"],
             "workload": f"{args.sur_serve} serving options (seed 4), 1024x1024 FD fallback"}
+
+    if args.peak_probe:
+        extra["device_ceilings"] = device_ceilings(achieved)
 
     cpu = None
     if args.cpu_sample > 0:

@@ -72,6 +72,11 @@ __global__ __launch_bounds__(256) void fd1d_lane_f64(FdBatch b, int rows) {
         }
     }
 
+    // a NaN or inf anywhere in the march ends non-finite in the unprojected last row
+    // (maxNum hides it in the projected rows): report NaN, as the reference's march would
+    bool bad = false;
+    for (int j = 0; j < (solve ? N : 0); ++j) bad |= !isfinite(at(kV, j));
+
     const uint32_t beg = b.off ? b.off[p] : 0u;
     const uint32_t end = b.off ? b.off[p + 1] : 1u;
     for (uint32_t k = beg; k < end; ++k) {
@@ -84,6 +89,8 @@ __global__ __launch_bounds__(256) void fd1d_lane_f64(FdBatch b, int rows) {
             const int j = bracket(P, b.density, xs);
             if (j == 0 || j == N) {
                 st = KWGPU_STATUS_OUT_OF_GRID;
+            } else if (bad) {
+                st = KWGPU_STATUS_NONFINITE;
             } else {
                 const double x1 = grid_x(P, b.density, j), x0 = grid_x(P, b.density, j - 1);
                 const double v = ((x1 - xs) * at(kV, j - 1) + (xs - x0) * at(kV, j)) / (x1 - x0);

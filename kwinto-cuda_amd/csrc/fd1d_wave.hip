@@ -275,28 +275,37 @@ __device__ __forceinline__ void setup(const FdBatch& b, const Pde& d, const Lane
     double bv[M];
     double bu_prev = 0.0;  // bu of row j0-1
     {
-        double xg[M + 3];  // x_{j0-2} .. x_{j0+M}
-        mesh_run(P, b.density, j0 - 2, xg);
-#pragma unroll
-        for (int i = 0; i < M + 3; ++i) {
-            const int j = j0 - 2 + i;
-            // nodes outside the mesh only feed padding rows; keep their spacings nonzero
-            if (j < 0 || j >= N) xg[i] = (j < 0 ? -1e6 : 1e6) + j;
-        }
-        double rdx[M + 2], rdm[M + 1];  // 1/(x_{i+1} - x_i), 1/(x_{i+2} - x_i) over xg
-#pragma unroll
-        for (int i = 0; i < M + 2; ++i) rdx[i] = rcp(xg[i + 1] - xg[i]);
-#pragma unroll
-        for (int i = 0; i < M + 1; ++i) rdm[i] = rcp(xg[i + 2] - xg[i]);
+        // nodes x_{j0-2} .. x_{j0+M} as a rolling window; nodes outside the mesh only
+        // feed padding rows and get far-away values that keep every spacing nonzero
+#pragma clang fp contract(off)
+        const double yj = (j0 - 2) * P.dy;
+        double e = exp(P.yMin * (1.0 - yj) + P.yMax * yj), f = rcp(e);
+        const double grow = exp((P.yMax - P.yMin) * P.dy), shrink = rcp(grow);
+        auto next_x = [&](int j) {
+            const double x = (j >= 0 && j < N) ? b.density * (0.5 * (e - f))
+                                               : (j < 0 ? -1e6 : 1e6) + j;
+            e *= grow;
+            f *= shrink;
+            return x;
+        };
+        double xa = next_x(j0 - 2), xb = next_x(j0 - 1), xc = next_x(j0);
+        double rl = rcp(xb - xa), ru = rcp(xc - xb);  // 1/(x_{j-1} - x_{j-2}), 1/(x_j - x_{j-1})
+        double rm = rcp(xc - xa);
         if (j0 >= 1 && j0 - 1 < N) {
             double t0, t1;
-            b_row_rcp(P, th, j0 - 1, rdx[0], rdx[1], rdm[0], t0, t1, bu_prev);
+            b_row_rcp(P, th, j0 - 1, rl, ru, rm, t0, t1, bu_prev);
         }
 #pragma unroll
         for (int i = 0; i < M; ++i) {
-            const int j = j0 + i;  // x_{j-1}, x_j, x_{j+1} = xg[i+1], xg[i+2], xg[i+3]
-            b_row_rcp(P, th, j, rdx[i + 1], rdx[i + 2], rdm[i + 1], blv[i], bv[i], buv[i]);
-            xv[i] = j < N ? payoff(P, xg[i + 2]) : 0.0;
+            const int j = j0 + i;  // window (x_{j-1}, x_j, x_{j+1}) = (xb, xc, xd)
+            const double xd = next_x(j + 1);
+            rl = ru;
+            ru = rcp(xd - xc);
+            rm = rcp(xd - xb);
+            b_row_rcp(P, th, j, rl, ru, rm, blv[i], bv[i], buv[i]);
+            xv[i] = j < N ? payoff(P, xc) : 0.0;
+            xb = xc;
+            xc = xd;
         }
     }
 
@@ -337,7 +346,7 @@ __device__ __forceinline__ void setup(const FdBatch& b, const Pde& d, const Lane
             e = t.lane == 0 ? w0 : map_mul(e, w0);
         }
     }
-    double dv[M], rdv[M];
+    double d0;  // this lane's first pivot
     {
         double dprev = t.ls == 0 ? 1.0 : (e.a + e.b) / (e.c + e.d);
         double bup = bu_prev;
@@ -348,23 +357,19 @@ __device__ __forceinline__ void setup(const FdBatch& b, const Pde& d, const Lane
             const double dd = bv[i] - blv[i] * (bup * rprev);
             const double rd = rcp(dd);
             lo[i] = blv[i] * rprev;  // bl = 0 on row 0, whatever the entering pivot
-            dv[i] = dd;
-            rdv[i] = rd;
+            if (i == 0) d0 = dd;
+            if (i > 0) up[i - 1] = buv[i - 1] * rd;  // u_j = bu_j / d_{j+1}
             ap[i] = inv_th * rd;
             rprev = rd;
             bup = buv[i];
         }
     }
-    // u_j = bu_j / d_{j+1}: the last row of a lane needs the next lane's first pivot
-    // (bu = 0 on the last real row and on padding rows, so the segment end needs none)
+    // the last row of a lane needs the next lane's first pivot (bu = 0 on the last real
+    // row and on padding rows, so the segment end needs none)
     __syncthreads();
-    qp[gl] = dv[0];
+    qp[gl] = d0;
     __syncthreads();
-    {
-        const double rnext = t.ls + 1 < LT ? rcp(qp[gl + 1]) : 1.0;
-#pragma unroll
-        for (int i = 0; i < M; ++i) up[i] = buv[i] * (i + 1 < M ? rdv[i + 1] : rnext);
-    }
+    up[M - 1] = buv[M - 1] * (t.ls + 1 < LT ? rcp(qp[gl + 1]) : 1.0);
 
     // Carry propagation coefficients: g_j = ghat_j + Pc_j g_in,
     // w_j = what_j + R_j g_in + Q_j w_in.
@@ -537,6 +542,22 @@ template <int M, int L, int W>
 __device__ __forceinline__ void interpolate(const FdBatch& b, const Pde& d, const Lanes& t,
                                             const double (&xv)[M], double* qp) {
     using Gm = Geo<M, L, W>;
+    // A NaN or inf anywhere in the march reaches the unprojected last row (the
+    // projection's maxNum turns NaN into the payoff elsewhere, -inf there), where the
+    // reference's own march would leave NaN: such an option is reported NaN / NONFINITE
+    // instead of as the payoff its projected rows show.
+    bool bad = false;
+#pragma unroll
+    for (int i = 0; i < M; ++i) bad |= d.j0 + i < d.N && !isfinite(xv[i]);
+    bool seg_bad;
+    if constexpr (W == 2) {
+        seg_bad = __syncthreads_or(bad);  // one option per workgroup
+    } else {
+        const uint64_t votes = __ballot(bad);
+        const int first = t.lane - t.ls;  // this option's first lane in the wave
+        const uint64_t mine = L == 64 ? ~0ull : (((1ull << L) - 1) << first);
+        seg_bad = (votes & mine) != 0;
+    }
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < M; ++i) qp[t.seg * Gm::LT * M + d.j0 + i] = xv[i];
@@ -555,6 +576,8 @@ __device__ __forceinline__ void interpolate(const FdBatch& b, const Pde& d, cons
             const int j = bracket(d.P, b.density, xs);
             if (j == 0 || j == d.N) {
                 st = KWGPU_STATUS_OUT_OF_GRID;
+            } else if (seg_bad) {
+                st = KWGPU_STATUS_NONFINITE;
             } else {
                 const double x1 = grid_x(d.P, b.density, j), x0 = grid_x(d.P, b.density, j - 1);
                 const double v = ((x1 - xs) * V[j - 1] + (xs - x0) * V[j]) / (x1 - x0);
@@ -628,13 +651,15 @@ __device__ __forceinline__ void solve_group_f64(const FdBatch& b, int group, dou
         qpv[i * NT + t.gl] = make_double2(Q[i], proj ? xv[i] : -INFINITY);
     }
     __syncthreads();
+    const int steps = d.steps;
     if (c_th == 1.0)
         march_f64<M, L, W, true>(xv, ap, lo, up, R, qpv, sm, t, first_lane, last_lane, xchg,
-                                 wsteps, d.steps, c_th);
+                                 wsteps, steps, c_th);
     else
         march_f64<M, L, W, false>(xv, ap, lo, up, R, qpv, sm, t, first_lane, last_lane, xchg,
-                                  wsteps, d.steps, c_th);
-    interpolate<M, L, W>(b, d, t, xv, qp);
+                                  wsteps, steps, c_th);
+    // the option's record is read again rather than kept in registers over the march
+    interpolate<M, L, W>(b, pde_of<M, L, W>(b, group, t), t, xv, qp);
 }
 
 // ---- FP32-mixed march ---------------------------------------------------------------
@@ -728,7 +753,8 @@ __device__ __forceinline__ void solve_group_mixed(const FdBatch& b, int group, d
             }
         }
     }
-    interpolate<M, L, W>(b, d, t, xv, qp);
+    // the option's record is read again rather than kept in registers over the march
+    interpolate<M, L, W>(b, pde_of<M, L, W>(b, group, t), t, xv, qp);
 }
 
 // M rows per lane, L lanes per option (W = 1) or W whole waves per option; WPE is the

@@ -267,24 +267,36 @@ __global__ __launch_bounds__(256) void gemm_bf16(GemmArgs g) {
 
 // output layer: y = b5 + W5 . silu(Z4) (FP32, one wave per sample); with targets,
 // dy = 2 (y - t) / n and the summed squared error.
-__global__ void out_fwd(const float* Z4, const float* p, int n, const float* target, float* y,
-                        float* dy, float* loss_sum) {
-    const int m = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
-    const int l = threadIdx.x & 63;
-    if (m >= n) return;
-    float s = 0.0f;
+// output layer: y = silu(Z4) W5 + b5; with targets also dy = 2 (y - t) / n and the loss
+// sum. kOutSPW samples per wave; the loss is reduced in the block before one atomic.
+constexpr int kOutSPW = 8;
+__global__ __launch_bounds__(256) void out_fwd(const float* Z4, const float* p, int n,
+                                               const float* target, float* y, float* dy,
+                                               float* loss_sum) {
+    __shared__ float part[4];
+    const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int m0 = (blockIdx.x * 4 + wave) * kOutSPW;
+    float loss = 0.0f;
+    for (int k = 0; k < kOutSPW; ++k) {
+        const int m = m0 + k;
+        if (m >= n) break;  // uniform per wave
+        float s = 0.0f;
 #pragma unroll
-    for (int k = l; k < H; k += 64) s = fmaf(p[kW5 + k], silu(Z4[(long)m * H + k]), s);
-    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-    if (l == 0) {
-        const float v = s + p[kB5];
-        y[m] = v;
-        if (target) {
-            const float d = v - target[m];
-            dy[m] = 2.0f * d / (float)n;
-            atomicAdd(loss_sum, d * d);
+        for (int c = l; c < H; c += 64) s = fmaf(p[kW5 + c], silu(Z4[(long)m * H + c]), s);
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+        if (l == 0) {
+            const float v = s + p[kB5];
+            y[m] = v;
+            if (target) {
+                const float d = v - target[m];
+                dy[m] = 2.0f * d / (float)n;
+                loss += d * d;
+            }
         }
     }
+    if (l == 0) part[wave] = loss;
+    __syncthreads();
+    if (target && threadIdx.x == 0) atomicAdd(loss_sum, part[0] + part[1] + part[2] + part[3]);
 }
 
 // dZ4 = dy W5 silu'(Z4) (BF16 for the next GEMMs)
@@ -297,14 +309,26 @@ __global__ void out_bwd(const float* Z4, const float* p, const float* dy, int n,
 
 // grad[kB? + n] += sum_m dZ[m][n]   and, for the output layer,
 // grad[kW5 + n] += sum_m dy[m] silu(Z4[m][n]), grad[kB5] += sum_m dy[m]
-__global__ void colsum_bf16(const bf16* dZ, int n, float* out) {
+// The column reductions below: 256 threads = 64 columns x 4 row phases; the four
+// phases are summed in LDS, so each column takes one atomic per block.
+__device__ __forceinline__ float phase_sum(float v, float* lds) {
+    lds[threadIdx.x] = v;
+    __syncthreads();
+    const int c = threadIdx.x & 63;
+    return lds[c] + lds[64 + c] + lds[128 + c] + lds[192 + c];
+}
+__global__ __launch_bounds__(256) void colsum_bf16(const bf16* dZ, int n, float* out) {
+    __shared__ float lds[256];
     const int col = blockIdx.x * 64 + (threadIdx.x & 63);
     const int r0 = blockIdx.y * 4 + (threadIdx.x >> 6);
     float s = 0.0f;
     for (int m = r0; m < n; m += 4 * gridDim.y) s += (float)dZ[(long)m * H + col];
-    atomicAdd(&out[col], s);
+    s = phase_sum(s, lds);
+    if (threadIdx.x < 64) atomicAdd(&out[col], s);
 }
-__global__ void out_grad(const float* Z4, const float* dy, int n, float* grad) {
+__global__ __launch_bounds__(256) void out_grad(const float* Z4, const float* dy, int n,
+                                                float* grad) {
+    __shared__ float lds[256];
     const int col = blockIdx.x * 64 + (threadIdx.x & 63);
     const int r0 = blockIdx.y * 4 + (threadIdx.x >> 6);
     float s = 0.0f, sb = 0.0f;
@@ -312,11 +336,18 @@ __global__ void out_grad(const float* Z4, const float* dy, int n, float* grad) {
         s = fmaf(dy[m], silu(Z4[(long)m * H + col]), s);
         if (col == 0) sb += dy[m];
     }
-    atomicAdd(&grad[kW5 + col], s);
-    if (col == 0) atomicAdd(&grad[kB5], sb);
+    s = phase_sum(s, lds);
+    __syncthreads();
+    sb = phase_sum(sb, lds);
+    if (threadIdx.x < 64) {
+        atomicAdd(&grad[kW5 + col], s);
+        if (col == 0) atomicAdd(&grad[kB5], sb);
+    }
 }
 // input layer gradients: grad W1[j][f] += sum_m dZ1[m][j] x[m][f], b1[j] += sum_m dZ1[m][j]
-__global__ void l1_grad(const bf16* dZ1, const float* x, int n, float* grad) {
+__global__ __launch_bounds__(256) void l1_grad(const bf16* dZ1, const float* x, int n,
+                                               float* grad) {
+    __shared__ float lds[256];
     const int col = blockIdx.x * 64 + (threadIdx.x & 63);
     const int r0 = blockIdx.y * 4 + (threadIdx.x >> 6);
     float s[F + 1] = {};
@@ -327,8 +358,11 @@ __global__ void l1_grad(const bf16* dZ1, const float* x, int n, float* grad) {
         s[F] += d;
     }
 #pragma unroll
-    for (int f = 0; f < F; ++f) atomicAdd(&grad[kW1 + col * F + f], s[f]);
-    atomicAdd(&grad[kB1 + col], s[F]);
+    for (int f = 0; f <= F; ++f) {
+        const float t = phase_sum(s[f], lds);
+        if (threadIdx.x < 64) atomicAdd(&grad[f < F ? kW1 + col * F + f : kB1 + col], t);
+        __syncthreads();
+    }
 }
 
 __global__ void adam_kernel(float* p, const float* g, float* m1, float* m2, int np, float lr,
@@ -508,8 +542,8 @@ hipError_t forward(kwgpu_surrogate* s, int mem, int n, float* yout, const float*
                               Z + (l + 1) * L, Hb + (l + 1) * L);
         if (hipError_t e = launch_gemm(EPI_FWD, g, st)) return e;
     }
-    hipLaunchKernelGGL(out_fwd, dim3(nblk(n, 4)), dim3(256), 0, st, Z + 3 * L, p, n, targets, yout,
-                       s->dy, s->loss);
+    hipLaunchKernelGGL(out_fwd, dim3(nblk(n, 4 * kOutSPW)), dim3(256), 0, st, Z + 3 * L, p, n,
+                       targets, yout, s->dy, s->loss);
     return hipGetLastError();
 }
 

@@ -173,7 +173,8 @@ def bench_uneven(ctx, torch, stream, n, steps, log):
         out[name] = {"options_per_s": n / (ms * 1e-3), "ms_per_step": ms,
                      "tflops": flops / (ms * 1e-3) / 1e12,
                      "roofline_frac": flops / (ms * 1e-3) / 1e12 / peak,
-                     "autotuned": {str(k): v for k, v in chosen.items()}}
+                     "autotuned": {str(k): v for k, v in chosen.items()},
+                     "bucket_schedule": tune.get("schedule")}
         log(f"bench: uneven {name} {ms:.3f} ms/step")
     d = np.abs(prices["fp32_mixed"] - prices["fp64"])
     out["fp32_mixed"]["max_abs_err_vs_fp64_fd"] = float(d.max())

@@ -255,8 +255,9 @@ int kwgpu_surrogate_set_params(kwgpu_ctx* ctx, kwgpu_surrogate* s, int32_t membe
                                const float* host_params);
 
 /* One forward/backward pass of `member` on normalised features (n x 6, FP32) and
- * scaled targets (n, FP32), both host memory: returns the mean squared error and its
- * gradient with respect to every parameter (host, KWGPU_SURROGATE_PARAMS values). */
+ * scaled targets (n, FP32), both host memory, n <= 131072: returns the mean squared
+ * error and its gradient with respect to every parameter (host, KWGPU_SURROGATE_PARAMS
+ * values). */
 int kwgpu_surrogate_loss_grad(kwgpu_ctx* ctx, kwgpu_surrogate* s, int32_t member,
                               const float* host_features, const float* host_targets, size_t n,
                               float* host_loss, float* host_grad);

@@ -94,13 +94,6 @@ __device__ __forceinline__ double vmax(double a, double b) {
     return r;
 }
 
-// Product of v[from..to] (inclusive); 1 when from > to.
-__device__ __forceinline__ double lds_prod(const double* v, int from, int to) {
-    double p = 1.0;
-    for (int k = from; k <= to; ++k) p *= v[k];
-    return p;
-}
-
 template <int M, int L, int W>
 struct Geo {
     static_assert(L == 16 || L == 32 || L == 64, "segment width");

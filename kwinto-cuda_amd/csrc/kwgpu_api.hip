@@ -49,13 +49,19 @@ struct kwgpu_ctx {
     std::string err;
     DevBuf opts, prices, status, rep, off, list, order, xd, td, ws, counters;
     DevBuf opts2, prices2, status2;  // second leg of FD1D-BS
-    // autotuned choices per (bucket, precision)
+    // autotuned choices per (bucket, precision), and per precision whether an uneven
+    // batch's buckets run one after another on the caller's stream (true) or
+    // concurrently on the side streams (false, the default)
     std::map<std::pair<int, int>, Choice> tuned;
+    std::map<int, bool> serial;
     // uneven batches: side streams, fork/join events, the cached plan and its graph
     hipStream_t side[kSide] = {};
     hipEvent_t ev_fork = nullptr, ev_join[kSide] = {};
     bool plan_valid = false;
-    uint64_t plan_key = 0;
+    uint64_t plan_key = 0;              // hash of everything but the sizes
+    std::vector<int32_t> plan_x, plan_t;  // the sizes the plan was made for (compared exactly)
+    int32_t* hsizes = nullptr;          // pinned staging for device-resident sizes
+    size_t hsizes_cap = 0;
     std::vector<Bucket> plan;
     hipGraphExec_t graph = nullptr;
     bool use_graphs = true;
@@ -163,22 +169,23 @@ int lane_launch(kwgpu_ctx* c, FdBatch bb, int max_x, int precision, hipStream_t 
     return KWGPU_OK;
 }
 
-// Launches the buckets of an uneven batch: bucket i on side stream i % kSide (forked
+// Launches the buckets of an uneven batch: in order on the caller's stream when
+// `serial`, else bucket i on side stream i % kSide (forked
 // from and joined back into s), persistent buckets taking work from counters[i].
 int launch_buckets(kwgpu_ctx* c, const FdBatch& b, const std::vector<Bucket>& plan, int precision,
-                   hipStream_t s) {
+                   hipStream_t s, bool serial) {
     hipError_t e;
     if ((e = kwgpu::launch_zero_counters((int*)c->counters.p, (int)plan.size(), s)))
         return hip_fail(c, e, "kwgpu: counter reset");
-    if ((e = hipEventRecord(c->ev_fork, s))) return hip_fail(c, e, "kwgpu: fork");
+    if (!serial && (e = hipEventRecord(c->ev_fork, s))) return hip_fail(c, e, "kwgpu: fork");
     size_t ws_off = 0;
     bool used[kSide] = {};
     for (size_t i = 0; i < plan.size(); ++i) {
         const Bucket& k = plan[i];
-        hipStream_t ss = c->side[i % kSide];
-        if (!used[i % kSide] && (e = hipStreamWaitEvent(ss, c->ev_fork, 0)))
+        hipStream_t ss = serial ? s : c->side[i % kSide];
+        if (!serial && !used[i % kSide] && (e = hipStreamWaitEvent(ss, c->ev_fork, 0)))
             return hip_fail(c, e, "kwgpu: fork wait");
-        used[i % kSide] = true;
+        if (!serial) used[i % kSide] = true;
         FdBatch bb = b;
         bb.order = (const uint32_t*)c->order.p + k.begin;
         bb.m = (int)k.count;
@@ -219,19 +226,20 @@ void drop_graph(kwgpu_ctx* c) {
 }
 
 // Runs the FD solve of `b` (device buffers already set) on stream s. pde_x / pde_t are
-// host copies of each PDE's grid sizes; empty means every PDE uses b.x_dim / b.t_dim.
+// host copies of each PDE's grid sizes (b.m each); null means every PDE uses b.x_dim /
+// b.t_dim.
 //
 // Uneven batches: PDEs are grouped by grid-size bucket, each bucket sorted by t-points
 // (descending) so that co-packed segments march for about the same number of steps and
 // a persistent grid takes the longest first; each bucket's kernel (autotuned variant)
 // runs on its own side stream. The whole launch sequence is captured once into a
 // hipGraph and replayed while the batch (sizes, buffers, parameters) is unchanged.
-int run_fd(kwgpu_ctx* c, FdBatch b, const std::vector<int32_t>& pde_x,
-           const std::vector<int32_t>& pde_t, int precision, int layout, hipStream_t s) {
-    const bool uniform = pde_x.empty();
+int run_fd(kwgpu_ctx* c, FdBatch b, const int32_t* pde_x, const int32_t* pde_t, int precision,
+           int layout, hipStream_t s) {
+    const bool uniform = pde_x == nullptr;
     if (precision == KWGPU_FP32_MIXED && layout == KWGPU_LAYOUT_LANE_SOA)
         return fail(c, KWGPU_ERR_UNSUPPORTED, "kwgpu: FP32-mixed runs in the wave layout only");
-    const int mx = uniform ? b.x_dim : *std::max_element(pde_x.begin(), pde_x.end());
+    const int mx = uniform ? b.x_dim : *std::max_element(pde_x, pde_x + b.m);
     if (layout == KWGPU_LAYOUT_WAVE_REG && !kwgpu::wave_layout_fits(mx, precision))
         return fail(c, KWGPU_ERR_UNSUPPORTED, "kwgpu: wave layout supports x_dim <= 2048");
     if (precision == KWGPU_FP32_MIXED && !kwgpu::wave_layout_fits(mx, precision))
@@ -254,28 +262,37 @@ int run_fd(kwgpu_ctx* c, FdBatch b, const std::vector<int32_t>& pde_x,
     }
 
     const int m = b.m;
-    std::vector<int> cap(m);
-    for (int p = 0; p < m; ++p) cap[p] = bucket_of(pde_x[p], precision, layout);
-    std::vector<uint32_t> ord(m);
-    std::iota(ord.begin(), ord.end(), 0u);
-    std::sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t z) {
-        return std::make_tuple(cap[a], -pde_t[a], a) < std::make_tuple(cap[z], -pde_t[z], z);
-    });
-    std::vector<Bucket> plan;
-    for (size_t i = 0; i < (size_t)m;) {
-        size_t j = i;
-        int bx = 0;
-        while (j < (size_t)m && cap[ord[j]] == cap[ord[i]]) bx = std::max(bx, pde_x[ord[j++]]);
-        plan.push_back(Bucket{cap[ord[i]], choice_for(c, cap[ord[i]], precision, layout), i, j - i, bx});
-        i = j;
+    // buckets in ascending capacity (64 .. 2048, then the lane layout): counts and widest
+    // grid, without sorting the PDEs
+    constexpr int kSlots = 7;
+    auto slot_of = [&](int x) {
+        const int cap = bucket_of(x, precision, layout);
+        return cap == kLaneCap ? kSlots - 1 : 31 - __builtin_clz((unsigned)cap) - 6;
+    };
+    size_t count[kSlots] = {};
+    int widest[kSlots] = {};
+    for (int p = 0; p < m; ++p) {
+        const int k = slot_of(pde_x[p]);
+        ++count[k];
+        widest[k] = std::max(widest[k], pde_x[p]);
     }
-    // the batch's identity: sizes, buffers, parameters and kernel choices
+    std::vector<Bucket> plan;
+    size_t begin = 0;
+    for (int k = 0; k < kSlots; ++k) {
+        if (!count[k]) continue;
+        const int cap = k == kSlots - 1 ? kLaneCap : 64 << k;
+        plan.push_back(Bucket{cap, choice_for(c, cap, precision, layout), begin, count[k], widest[k]});
+        begin += count[k];
+    }
+    // the batch's identity: buffers, parameters and kernel choices (hashed) and the sizes
+    // (compared exactly with the plan's own copy)
     uint64_t key = 1469598103934665603ull;
-    key = fnv(key, pde_x.data(), pde_x.size() * sizeof(int32_t));
-    key = fnv(key, pde_t.data(), pde_t.size() * sizeof(int32_t));
     key = fnv(key, &b, sizeof(b));
     key = fnv(key, &precision, sizeof(precision));
     for (const Bucket& k : plan) key = fnv(key, &k, sizeof(k));
+    const auto sit = c->serial.find(precision);
+    const bool serial = sit != c->serial.end() && sit->second;
+    key = fnv(key, &serial, sizeof(serial));
 
     // ... and the context buffers a captured graph would reference
     auto with_buffers = [&](uint64_t k) {
@@ -283,7 +300,10 @@ int run_fd(kwgpu_ctx* c, FdBatch b, const std::vector<int32_t>& pde_x,
         return fnv(k, bufs, sizeof(bufs));
     };
     if (int rc = ensure_side_streams(c)) return rc;
-    const bool rebuild = !c->plan_valid || c->plan_key != with_buffers(key);
+    const size_t bytes = sizeof(int32_t) * (size_t)m;
+    const bool rebuild = !c->plan_valid || c->plan_key != with_buffers(key) ||
+                         c->plan_x.size() != (size_t)m || std::memcmp(c->plan_x.data(), pde_x, bytes) ||
+                         std::memcmp(c->plan_t.data(), pde_t, bytes);
     if (rebuild) {
         drop_graph(c);
         size_t ws = 0;
@@ -294,20 +314,30 @@ int run_fd(kwgpu_ctx* c, FdBatch b, const std::vector<int32_t>& pde_x,
             (ws && ensure(c->ws, ws * sizeof(double)) != hipSuccess))
             return fail(c, KWGPU_ERR_NOMEM, "kwgpu: uneven batch allocation failed");
         key = with_buffers(key);
+        // PDEs by bucket, longest march first within a bucket (ties by index)
+        std::vector<int> slot(m);
+        for (int p = 0; p < m; ++p) slot[p] = slot_of(pde_x[p]);
+        std::vector<uint32_t> ord(m);
+        std::iota(ord.begin(), ord.end(), 0u);
+        std::sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t z) {
+            return std::make_tuple(slot[a], -pde_t[a], a) < std::make_tuple(slot[z], -pde_t[z], z);
+        });
         hipError_t e;
         if ((e = hipMemcpyAsync(c->order.p, ord.data(), sizeof(uint32_t) * m, hipMemcpyHostToDevice, s)) ||
             (e = hipStreamSynchronize(s)))
             return hip_fail(c, e, "kwgpu: order upload");
         c->plan = plan;
         c->plan_key = key;
+        c->plan_x.assign(pde_x, pde_x + m);
+        c->plan_t.assign(pde_t, pde_t + m);
         c->plan_valid = true;
     }
-    if (!c->use_graphs) return launch_buckets(c, b, c->plan, precision, s);
+    if (!c->use_graphs) return launch_buckets(c, b, c->plan, precision, s, serial);
     if (!c->graph) {
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
         if (e == hipSuccess) {
-            const int rc = launch_buckets(c, b, c->plan, precision, s);
+            const int rc = launch_buckets(c, b, c->plan, precision, s, serial);
             hipError_t e2 = hipStreamEndCapture(s, &g);
             if (rc) {
                 if (g) (void)hipGraphDestroy(g);
@@ -321,7 +351,7 @@ int run_fd(kwgpu_ctx* c, FdBatch b, const std::vector<int32_t>& pde_x,
             // streams that cannot be captured (e.g. the legacy null stream): launch directly
             (void)hipGetLastError();
             c->graph = nullptr;
-            return launch_buckets(c, b, c->plan, precision, s);
+            return launch_buckets(c, b, c->plan, precision, s, serial);
         }
     }
     if (hipError_t e = hipGraphLaunch(c->graph, s); e != hipSuccess)
@@ -409,7 +439,8 @@ int fd_price_host(kwgpu_ctx* c, const kwgpu_option* opts, size_t n, const kwgpu_
         // the host vectors must outlive the async copies
         if ((e = hipStreamSynchronize(s)) != hipSuccess) return hip_fail(c, e, "kwgpu: sync");
     }
-    int rc = run_fd(c, b, pde_x, pde_t, p->precision, p->layout, s);
+    int rc = run_fd(c, b, pde_x.empty() ? nullptr : pde_x.data(),
+                    pde_t.empty() ? nullptr : pde_t.data(), p->precision, p->layout, s);
     // the host vectors above must outlive every copy queued from them
     e = hipStreamSynchronize(s);
     if (rc) return rc;
@@ -503,6 +534,7 @@ void kwgpu_destroy(kwgpu_ctx* c) {
         if (c->ev_join[k]) (void)hipEventDestroy(c->ev_join[k]);
     }
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    if (c->hsizes) (void)hipHostFree(c->hsizes);
     for (DevBuf* b : {&c->opts, &c->prices, &c->status, &c->rep, &c->off, &c->list, &c->order,
                       &c->xd, &c->td, &c->ws, &c->counters, &c->opts2, &c->prices2, &c->status2})
         free_buf(*b);
@@ -564,21 +596,30 @@ int kwgpu_fd1d_price_device(kwgpu_ctx* c, const kwgpu_option* d_opts, size_t n,
     b.theta = p->theta;
     b.prices = d_prices;
     b.status = d_status;
-    std::vector<int32_t> hx, ht;
-    if (p->x_dims || p->t_dims) {
-        // grouping needs the sizes on the host
-        hx.assign(n, p->x_dim);
-        ht.assign(n, p->t_dim);
-        hipError_t e;
-        if (p->x_dims &&
-            (e = hipMemcpyAsync(hx.data(), p->x_dims, n * sizeof(int32_t), hipMemcpyDeviceToHost, s)))
-            return hip_fail(c, e, "kwgpu: x_dims download");
-        if (p->t_dims &&
-            (e = hipMemcpyAsync(ht.data(), p->t_dims, n * sizeof(int32_t), hipMemcpyDeviceToHost, s)))
-            return hip_fail(c, e, "kwgpu: t_dims download");
-        if ((e = hipStreamSynchronize(s)) != hipSuccess) return hip_fail(c, e, "kwgpu: sync");
-        if (int rc = check_dims(c, hx.data(), ht.data(), n)) return rc;
+    if (!p->x_dims && !p->t_dims) return run_fd(c, b, nullptr, nullptr, p->precision, p->layout, s);
+    // grouping needs the sizes on the host: pinned staging owned by the context
+    if (c->hsizes_cap < 2 * n) {
+        if (c->hsizes) (void)hipHostFree(c->hsizes);
+        c->hsizes = nullptr;
+        c->hsizes_cap = 0;
+        if (hipHostMalloc((void**)&c->hsizes, 2 * n * sizeof(int32_t), hipHostMallocDefault) !=
+            hipSuccess)
+            return fail(c, KWGPU_ERR_NOMEM, "kwgpu: pinned staging allocation failed");
+        c->hsizes_cap = 2 * n;
     }
+    int32_t* hx = c->hsizes;
+    int32_t* ht = c->hsizes + n;
+    hipError_t e;
+    if (!p->x_dims) std::fill(hx, hx + n, p->x_dim);
+    if (!p->t_dims) std::fill(ht, ht + n, p->t_dim);
+    if (p->x_dims &&
+        (e = hipMemcpyAsync(hx, p->x_dims, n * sizeof(int32_t), hipMemcpyDeviceToHost, s)))
+        return hip_fail(c, e, "kwgpu: x_dims download");
+    if (p->t_dims &&
+        (e = hipMemcpyAsync(ht, p->t_dims, n * sizeof(int32_t), hipMemcpyDeviceToHost, s)))
+        return hip_fail(c, e, "kwgpu: t_dims download");
+    if ((e = hipStreamSynchronize(s)) != hipSuccess) return hip_fail(c, e, "kwgpu: sync");
+    if (int rc = check_dims(c, hx, ht, n)) return rc;
     return run_fd(c, b, hx, ht, p->precision, p->layout, s);
 }
 
@@ -797,12 +838,17 @@ int kwgpu_fd1d_autotune(kwgpu_ctx* c, const kwgpu_option* d_opts, size_t n,
             if (kwgpu::wave_variant(v).capacity() == cap)
                 for (bool per : {false, true}) cands.push_back({Choice{v, per}, 0.f});
         if (prec == KWGPU_FP64) cands.push_back({Choice{-1, false}, 0.f});
+        // the lane layout streams its state through HBM and runs ~100x slower than the
+        // wave variants: it is timed on the first kLaneSample sampled options and scaled
+        constexpr size_t kLaneSample = 256;
+        FdBatch bl = bb;
+        bl.m = (int)std::min(S, kLaneSample);
         for (auto& cd : cands) {
             auto run = [&]() -> int {
                 if (cd.ch.variant < 0) {
-                    if (ensure(ws, kwgpu::lane_workspace_doubles(bb.m, bx) * sizeof(double)))
+                    if (ensure(ws, kwgpu::lane_workspace_doubles(bl.m, bx) * sizeof(double)))
                         return fail(c, KWGPU_ERR_NOMEM, "kwgpu_fd1d_autotune: workspace");
-                    return lane_launch(c, bb, bx, prec, s, (double*)ws.p);
+                    return lane_launch(c, bl, bx, prec, s, (double*)ws.p);
                 }
                 int* q = nullptr;
                 if (cd.ch.persist) {
@@ -814,15 +860,17 @@ int kwgpu_fd1d_autotune(kwgpu_ctx* c, const kwgpu_option* d_opts, size_t n,
                 return KWGPU_OK;
             };
             if ((rc = run())) break;  // warm-up
+            const int reps = cd.ch.variant < 0 ? 1 : 2;
             if ((e = hipEventRecord(e0, s))) { rc = hip_fail(c, e, "event"); break; }
-            for (int r = 0; r < 2 && !rc; ++r) rc = run();
+            for (int r = 0; r < reps && !rc; ++r) rc = run();
             if (rc) break;
             if ((e = hipEventRecord(e1, s)) || (e = hipEventSynchronize(e1)) ||
                 (e = hipEventElapsedTime(&cd.ms, e0, e1))) {
                 rc = hip_fail(c, e, "event");
                 break;
             }
-            cd.ms /= 2;
+            cd.ms /= reps;
+            if (cd.ch.variant < 0) cd.ms *= (float)S / (float)bl.m;
         }
         if (rc) break;
         const Cand* best = &cands[0];
@@ -851,7 +899,40 @@ int kwgpu_fd1d_autotune(kwgpu_ctx* c, const kwgpu_option* d_opts, size_t n,
         std::snprintf(buf, sizeof(buf), "], \"chosen\": %d}", (int)(best - &cands[0]));
         js += buf;
     }
-    js += "]}";
+    js += "]";
+    // the bucket schedule: the whole batch with the chosen kernels, buckets concurrent on
+    // the side streams and one after another on the caller's stream
+    if (rc == KWGPU_OK && buckets.size() > 1) {
+        b.m = (int)n;
+        const bool graphs = c->use_graphs;
+        c->use_graphs = false;
+        float ms[2] = {0.f, 0.f};
+        for (int serial = 0; serial < 2 && rc == KWGPU_OK; ++serial) {
+            c->serial[prec] = serial != 0;
+            if ((rc = run_fd(c, b, hx.data(), ht.data(), prec, KWGPU_LAYOUT_AUTO, s))) break;
+            if ((e = hipEventRecord(e0, s))) { rc = hip_fail(c, e, "event"); break; }
+            for (int r = 0; r < 2 && rc == KWGPU_OK; ++r)
+                rc = run_fd(c, b, hx.data(), ht.data(), prec, KWGPU_LAYOUT_AUTO, s);
+            if (rc) break;
+            if ((e = hipEventRecord(e1, s)) || (e = hipEventSynchronize(e1)) ||
+                (e = hipEventElapsedTime(&ms[serial], e0, e1))) {
+                rc = hip_fail(c, e, "event");
+                break;
+            }
+            ms[serial] /= 2;
+        }
+        c->use_graphs = graphs;
+        if (rc == KWGPU_OK) {
+            c->serial[prec] = ms[1] < ms[0];
+            char buf[160];
+            std::snprintf(buf, sizeof(buf),
+                          ", \"schedule\": {\"concurrent_ms\": %.4f, \"serial_ms\": %.4f, "
+                          "\"chosen\": \"%s\"}",
+                          ms[0], ms[1], ms[1] < ms[0] ? "serial" : "concurrent");
+            js += buf;
+        }
+    }
+    js += "}";
     (void)hipStreamSynchronize(s);
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
@@ -869,6 +950,8 @@ int kwgpu_tune_save(kwgpu_ctx* c, const char* path) {
     if (!f) return fail(c, KWGPU_ERR_INVALID, std::string("cannot write ") + path);
     for (const auto& [k, ch] : c->tuned)
         std::fprintf(f, "%d %d %d %d\n", k.first, k.second, ch.variant, ch.persist ? 1 : 0);
+    for (const auto& [prec, serial] : c->serial)  // bucket -1: the bucket schedule
+        std::fprintf(f, "%d %d %d %d\n", -1, prec, serial ? 1 : 0, 0);
     std::fclose(f);
     return KWGPU_OK;
 }
@@ -879,6 +962,10 @@ int kwgpu_tune_load(kwgpu_ctx* c, const char* path) {
     if (!f) return fail(c, KWGPU_ERR_INVALID, std::string("cannot read ") + path);
     int cap, prec, v, per;
     while (std::fscanf(f, "%d %d %d %d", &cap, &prec, &v, &per) == 4) {
+        if (cap == -1) {
+            c->serial[prec] = v != 0;
+            continue;
+        }
         if (v >= kwgpu::wave_variant_count() || (v >= 0 && kwgpu::wave_variant(v).capacity() != cap)) continue;
         c->tuned[{cap, prec}] = Choice{v, per != 0};
     }

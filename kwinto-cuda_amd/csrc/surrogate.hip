@@ -137,13 +137,18 @@ __global__ void features_kernel(const kwgpu_option* opts, size_t n, const double
     if (in_box) in_box[i] = in ? 1 : 0;
 }
 
-__global__ void gather_kernel(const float* feat, const float* target, const uint32_t* idx, int n,
-                              float* bfeat, float* btarget) {
+// Member-batched kernels: the pointer arguments are the first member's; member k's
+// data follows at the given element strides, k = blockIdx.y (blockIdx.z for 2-D grids).
+
+// batch rows of member k: its permutation idx + k * sidx picks the samples
+__global__ void gather_kernel(const float* feat, const float* target, const uint32_t* idx,
+                              long sidx, int n, float* bfeat, float* btarget, long srow) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const uint32_t s = idx[i];
-    for (int k = 0; k < F; ++k) bfeat[i * F + k] = feat[(size_t)s * F + k];
-    btarget[i] = target[s];
+    const int k = blockIdx.y;
+    const uint32_t s = idx[k * sidx + i];
+    for (int f = 0; f < F; ++f) bfeat[k * srow * F + i * F + f] = feat[(size_t)s * F + f];
+    btarget[k * srow + i] = target[s];
 }
 
 __global__ void baseline_kernel(const kwgpu_option* opts, size_t n, double* out) {
@@ -153,9 +158,15 @@ __global__ void baseline_kernel(const kwgpu_option* opts, size_t n, double* out)
 
 // ---- layers ----------------------------------------------------------------------
 // input layer in FP32: Z1 = x W1^T + b1, H1 = silu(Z1) in BF16
-__global__ void l1_fwd(const float* x, const float* p, int n, float* Z, bf16* Hb) {
+__global__ void l1_fwd(const float* x, long sx, const float* p, int n, float* Z, bf16* Hb,
+                       long sz) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n * H) return;
+    const int k = blockIdx.y;
+    x += k * sx;
+    p += (long)k * kP;
+    Z += k * sz;
+    Hb += k * sz;
     const int m = i / H, j = i - m * H;
     float z = p[kB1 + j];
 #pragma unroll
@@ -176,7 +187,9 @@ struct GemmArgs {
     float* Z;           // EPI_FWD: writes pre-activations; EPI_BWD: reads them
     bf16* Hout;         // EPI_FWD: silu(Z); EPI_BWD: dZ = C * silu'(Z)
     float* C;           // EPI_WGRAD: C[m * N + n] += sum over this block's K range
-    int k_split;        // EPI_WGRAD: K range per blockIdx.z
+    int k_split;        // EPI_WGRAD: K range per split
+    int splits;         // EPI_WGRAD: K splits; blockIdx.z = member * splits + split
+    long mA, mB, mBias, mZ, mH, mC;  // member strides (elements)
 };
 
 // C(M x N) = A(M x K) B(K x N), 64 x 64 tile per 256-thread workgroup, each wave a
@@ -189,9 +202,19 @@ __global__ __launch_bounds__(256) void gemm_bf16(GemmArgs g) {
     const int t = threadIdx.x, w = t >> 6, l = t & 63;
     const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
     int kb = 0, ke = g.K;
+    const int mem = EPI == EPI_WGRAD ? blockIdx.z / g.splits : blockIdx.z;
     if constexpr (EPI == EPI_WGRAD) {
-        kb = blockIdx.z * g.k_split;
+        kb = (blockIdx.z - mem * g.splits) * g.k_split;
         ke = min(g.K, kb + g.k_split);
+    }
+    g.A += mem * g.mA;
+    g.B += mem * g.mB;
+    if constexpr (EPI == EPI_FWD) g.bias += mem * g.mBias;
+    if constexpr (EPI != EPI_WGRAD) {
+        g.Z += mem * g.mZ;
+        g.Hout += mem * g.mH;
+    } else {
+        g.C += mem * g.mC;
     }
     const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
     floatx4 acc[2][2];
@@ -201,6 +224,29 @@ __global__ __launch_bounds__(256) void gemm_bf16(GemmArgs g) {
         for (int j = 0; j < 2; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
     const int lr = t >> 2, lc = (t & 3) * 8;  // tile row 0..63, k offset 0/8/16/24
     for (int k0 = kb; k0 < ke; k0 += BK) {
+        if constexpr (EPI == EPI_WGRAD) {
+            // both operands have K (the batch) across rows and M / N along them: load 16
+            // bytes along the row and transpose into the K-contiguous LDS tiles
+            const int kk = t >> 3, c8 = (t & 7) * 8;  // k row 0..31, column group 0..56
+            const int k = k0 + kk;
+            bf16x8 va, vb;
+            if (g.sam == 1 && g.sbn == 1 && k < ke && m0 + c8 + 8 <= g.M && n0 + c8 + 8 <= g.N) {
+                va = *reinterpret_cast<const bf16x8*>(g.A + (long)k * g.sak + m0 + c8);
+                vb = *reinterpret_cast<const bf16x8*>(g.B + (long)k * g.sbk + n0 + c8);
+            } else {
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    const int m = m0 + c8 + i, n = n0 + c8 + i;
+                    va[i] = (k < ke && m < g.M) ? g.A[(long)m * g.sam + (long)k * g.sak] : (bf16)0.0f;
+                    vb[i] = (k < ke && n < g.N) ? g.B[(long)k * g.sbk + (long)n * g.sbn] : (bf16)0.0f;
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                As[(c8 + i) * LD + kk] = va[i];
+                Bs[(c8 + i) * LD + kk] = vb[i];
+            }
+        } else {
         {
             const int m = m0 + lr, k = k0 + lc;
             bf16x8 v;
@@ -226,6 +272,7 @@ __global__ __launch_bounds__(256) void gemm_bf16(GemmArgs g) {
                                                    : (bf16)0.0f;
             }
             *reinterpret_cast<bf16x8*>(&Bs[lr * LD + lc]) = v;
+        }
         }
         __syncthreads();
 #pragma unroll
@@ -270,11 +317,22 @@ __global__ __launch_bounds__(256) void gemm_bf16(GemmArgs g) {
 // output layer: y = silu(Z4) W5 + b5; with targets also dy = 2 (y - t) / n and the loss
 // sum. kOutSPW samples per wave; the loss is reduced in the block before one atomic.
 constexpr int kOutSPW = 8;
-__global__ __launch_bounds__(256) void out_fwd(const float* Z4, const float* p, int n,
-                                               const float* target, float* y, float* dy,
-                                               float* loss_sum) {
+__global__ __launch_bounds__(256) void out_fwd(const float* Z4, long sz, const float* p, int n,
+                                               const float* target, float* y, long sy, float* dy,
+                                               long srow, float* loss_sum) {
     __shared__ float part[4];
     const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
+    {
+        const int k = blockIdx.y;
+        Z4 += k * sz;
+        p += (long)k * kP;
+        y += k * sy;
+        if (target) {
+            target += k * srow;
+            dy += k * srow;
+            loss_sum += k;
+        }
+    }
     const int m0 = (blockIdx.x * 4 + wave) * kOutSPW;
     float loss = 0.0f;
     for (int k = 0; k < kOutSPW; ++k) {
@@ -300,11 +358,17 @@ __global__ __launch_bounds__(256) void out_fwd(const float* Z4, const float* p, 
 }
 
 // dZ4 = dy W5 silu'(Z4) (BF16 for the next GEMMs)
-__global__ void out_bwd(const float* Z4, const float* p, const float* dy, int n, bf16* dZ) {
+__global__ void out_bwd(const float* Z4, long sz, const float* p, const float* dy, long srow, int n,
+                        bf16* dZ, long sdz) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n * H) return;
-    const int m = i / H, k = i - m * H;
-    dZ[i] = (bf16)(dy[m] * p[kW5 + k] * dsilu(Z4[i]));
+    const int mem = blockIdx.y;
+    Z4 += mem * sz;
+    p += (long)mem * kP;
+    dy += mem * srow;
+    dZ += mem * sdz;
+    const int m = i / H, c = i - m * H;
+    dZ[i] = (bf16)(dy[m] * p[kW5 + c] * dsilu(Z4[i]));
 }
 
 // grad[kB? + n] += sum_m dZ[m][n]   and, for the output layer,
@@ -317,18 +381,24 @@ __device__ __forceinline__ float phase_sum(float v, float* lds) {
     const int c = threadIdx.x & 63;
     return lds[c] + lds[64 + c] + lds[128 + c] + lds[192 + c];
 }
-__global__ __launch_bounds__(256) void colsum_bf16(const bf16* dZ, int n, float* out) {
+__global__ __launch_bounds__(256) void colsum_bf16(const bf16* dZ, long sdz, int n, float* out) {
     __shared__ float lds[256];
+    dZ += blockIdx.z * sdz;
+    out += (long)blockIdx.z * kP;
     const int col = blockIdx.x * 64 + (threadIdx.x & 63);
     const int r0 = blockIdx.y * 4 + (threadIdx.x >> 6);
     float s = 0.0f;
+#pragma unroll 4
     for (int m = r0; m < n; m += 4 * gridDim.y) s += (float)dZ[(long)m * H + col];
     s = phase_sum(s, lds);
     if (threadIdx.x < 64) atomicAdd(&out[col], s);
 }
-__global__ __launch_bounds__(256) void out_grad(const float* Z4, const float* dy, int n,
-                                                float* grad) {
+__global__ __launch_bounds__(256) void out_grad(const float* Z4, long sz, const float* dy,
+                                                long srow, int n, float* grad) {
     __shared__ float lds[256];
+    Z4 += blockIdx.z * sz;
+    dy += blockIdx.z * srow;
+    grad += (long)blockIdx.z * kP;
     const int col = blockIdx.x * 64 + (threadIdx.x & 63);
     const int r0 = blockIdx.y * 4 + (threadIdx.x >> 6);
     float s = 0.0f, sb = 0.0f;
@@ -345,9 +415,12 @@ __global__ __launch_bounds__(256) void out_grad(const float* Z4, const float* dy
     }
 }
 // input layer gradients: grad W1[j][f] += sum_m dZ1[m][j] x[m][f], b1[j] += sum_m dZ1[m][j]
-__global__ __launch_bounds__(256) void l1_grad(const bf16* dZ1, const float* x, int n,
-                                               float* grad) {
+__global__ __launch_bounds__(256) void l1_grad(const bf16* dZ1, long sdz, const float* x, long sx,
+                                               int n, float* grad) {
     __shared__ float lds[256];
+    dZ1 += blockIdx.z * sdz;
+    x += blockIdx.z * sx;
+    grad += (long)blockIdx.z * kP;
     const int col = blockIdx.x * 64 + (threadIdx.x & 63);
     const int r0 = blockIdx.y * 4 + (threadIdx.x >> 6);
     float s[F + 1] = {};
@@ -369,6 +442,11 @@ __global__ void adam_kernel(float* p, const float* g, float* m1, float* m2, int 
                             float b1, float b2, float eps, float c1, float c2) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= np) return;
+    const long o = (long)blockIdx.y * kP;  // member
+    p += o;
+    g += o;
+    m1 += o;
+    m2 += o;
     const float gi = g[i];
     const float a = b1 * m1[i] + (1.0f - b1) * gi;
     const float v = b2 * m2[i] + (1.0f - b2) * gi * gi;
@@ -377,9 +455,12 @@ __global__ void adam_kernel(float* p, const float* g, float* m1, float* m2, int 
     p[i] -= lr * (a / c1) / (sqrtf(v / c2) + eps);
 }
 
-__global__ void to_bf16(const float* p, bf16* w, int n) {
+// BF16 copies of W2..W4: blockIdx.y = 3 * member + layer
+__global__ void to_bf16(const float* p, bf16* w) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) w[i] = (bf16)p[i];
+    if (i >= H * H) return;
+    const int k = blockIdx.y / 3, l = blockIdx.y - 3 * k;
+    w[(long)k * 3 * H * H + (long)l * H * H + i] = (bf16)p[(long)k * kP + kW2 + l * (H * H + H) + i];
 }
 
 __global__ void zero_f(float* p, size_t n) {
@@ -427,8 +508,19 @@ __global__ void gate_kernel(const kwgpu_option* opts, size_t n, const double* pr
         r = 1;
     }
     if (r != 0) r = (o.z * sqrt(o.t) > fp64_above) ? 2 : 1;
-    if (r == 1) idx1[atomicAdd(&counts[0], 1)] = (uint32_t)i;
-    if (r == 2) idx2[atomicAdd(&counts[1], 1)] = (uint32_t)i;
+    // compaction with one atomic per wave and route
+    const int lane = threadIdx.x & 63;
+    const uint64_t below = (1ull << lane) - 1;
+#pragma unroll
+    for (int k = 1; k <= 2; ++k) {
+        const uint64_t votes = __ballot(r == k);
+        if (votes == 0) continue;
+        const int leader = __ffsll((long long)votes) - 1;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&counts[k - 1], __popcll(votes));
+        base = __shfl(base, leader);
+        if (r == k) (k == 1 ? idx1 : idx2)[base + __popcll(votes & below)] = (uint32_t)i;
+    }
     prices[i] = price;
     route[i] = r;
 }
@@ -455,13 +547,15 @@ using namespace kwgpu::sur;
 struct kwgpu_surrogate {
     int members = 0;
     int device = 0;
-    std::vector<float*> p;             // FP32 master weights per member
-    std::vector<bf16*> wb;             // BF16 W2 | W3 | W4 per member
-    // scratch (grown on demand)
+    float* pall = nullptr;             // FP32 master weights, member-major (kP each)
+    bf16* wball = nullptr;             // BF16 W2 | W3 | W4, member-major (3 H^2 each)
+    std::vector<float*> p;             // pall + m kP
+    std::vector<bf16*> wb;             // wball + 3 m H^2
+    float *grad = nullptr, *m1 = nullptr, *m2 = nullptr, *loss = nullptr;  // per member
+    // per-member scratch slots of `cap` rows each (grown on demand)
     size_t cap = 0;
     float *x = nullptr, *tg = nullptr, *y = nullptr, *dy = nullptr, *Z = nullptr;
     bf16 *Hb = nullptr, *dZa = nullptr, *dZb = nullptr;
-    float *grad = nullptr, *m1 = nullptr, *m2 = nullptr, *loss = nullptr;
     uint32_t* idx = nullptr;
     float* yall = nullptr;
     size_t yall_cap = 0;
@@ -489,19 +583,21 @@ void free_scratch(kwgpu_surrogate* s) {
     s->cap = 0;
 }
 
-// batch scratch for `rows` samples: x, targets, y, dy, Z for 4 layers, H for 4 layers,
-// two dZ buffers
+// batch scratch, one slot of `rows` samples per member: x, targets, y, dy, Z and H for 4
+// layers, two dZ buffers
 hipError_t ensure_scratch(kwgpu_surrogate* s, size_t rows) {
     if (s->cap >= rows) return hipSuccess;
     free_scratch(s);
+    const size_t M = (size_t)s->members;
     hipError_t e;
-    if ((e = hipMalloc(&s->x, rows * F * sizeof(float))) ||
-        (e = hipMalloc(&s->tg, rows * sizeof(float))) || (e = hipMalloc(&s->y, rows * sizeof(float))) ||
-        (e = hipMalloc(&s->dy, rows * sizeof(float))) ||
-        (e = hipMalloc(&s->Z, 4 * rows * H * sizeof(float))) ||
-        (e = hipMalloc(&s->Hb, 4 * rows * H * sizeof(bf16))) ||
-        (e = hipMalloc(&s->dZa, rows * H * sizeof(bf16))) ||
-        (e = hipMalloc(&s->dZb, rows * H * sizeof(bf16))) ||
+    if ((e = hipMalloc(&s->x, M * rows * F * sizeof(float))) ||
+        (e = hipMalloc(&s->tg, M * rows * sizeof(float))) ||
+        (e = hipMalloc(&s->y, M * rows * sizeof(float))) ||
+        (e = hipMalloc(&s->dy, M * rows * sizeof(float))) ||
+        (e = hipMalloc(&s->Z, M * 4 * rows * H * sizeof(float))) ||
+        (e = hipMalloc(&s->Hb, M * 4 * rows * H * sizeof(bf16))) ||
+        (e = hipMalloc(&s->dZa, M * rows * H * sizeof(bf16))) ||
+        (e = hipMalloc(&s->dZb, M * rows * H * sizeof(bf16))) ||
         (e = hipMalloc(&s->idx, rows * sizeof(uint32_t))))
         return e;
     s->cap = rows;
@@ -517,8 +613,8 @@ GemmArgs gemm_fwd(const bf16* Hin, const bf16* W, int n, const float* bias, floa
     return g;
 }
 
-hipError_t launch_gemm(int epi, const GemmArgs& g, hipStream_t st, int splits = 1) {
-    dim3 grid(nblk(g.N, 64), nblk(g.M, 64), splits);
+hipError_t launch_gemm(int epi, const GemmArgs& g, hipStream_t st, int z) {
+    dim3 grid(nblk(g.N, 64), nblk(g.M, 64), z);
     if (epi == EPI_FWD)
         hipLaunchKernelGGL(gemm_bf16<EPI_FWD>, grid, dim3(256), 0, st, g);
     else if (epi == EPI_BWD)
@@ -528,36 +624,50 @@ hipError_t launch_gemm(int epi, const GemmArgs& g, hipStream_t st, int splits = 
     return hipGetLastError();
 }
 
-// forward of one member on s->x (n rows); y into `yout`; with targets also dy and loss
-hipError_t forward(kwgpu_surrogate* s, int mem, int n, float* yout, const float* targets,
-                   hipStream_t st) {
-    const float* p = s->p[mem];
-    const bf16* wb = s->wb[mem];
+// Forward of members m0 .. m0+cnt-1 (parameters), scratch slots 0 .. cnt-1, n rows each:
+// features from s->x (slot k, or slot 0 for every member when shared_x); y into
+// yout + k sy; with targets (s->tg) also dy and the loss sums s->loss[k].
+hipError_t forward(kwgpu_surrogate* s, int m0, int cnt, int n, bool shared_x, float* yout,
+                   long sy, bool with_targets, hipStream_t st) {
+    const float* p = s->p[m0];
+    const bf16* wb = s->wb[m0];
+    const long cap = (long)s->cap, sz = 4 * cap * H, sx = shared_x ? 0 : cap * F;
     float* Z = s->Z;
     bf16* Hb = s->Hb;
     const size_t L = (size_t)n * H;
-    hipLaunchKernelGGL(l1_fwd, dim3(nblk(L, 256)), dim3(256), 0, st, s->x, p, n, Z, Hb);
+    hipLaunchKernelGGL(l1_fwd, dim3(nblk(L, 256), cnt), dim3(256), 0, st, s->x, sx, p, n, Z, Hb, sz);
     for (int l = 0; l < 3; ++l) {
         GemmArgs g = gemm_fwd(Hb + l * L, wb + l * H * H, n, p + kB2 + l * (H * H + H),
                               Z + (l + 1) * L, Hb + (l + 1) * L);
-        if (hipError_t e = launch_gemm(EPI_FWD, g, st)) return e;
+        g.mA = sz;
+        g.mB = 3 * H * H;
+        g.mBias = kP;
+        g.mZ = sz;
+        g.mH = sz;
+        if (hipError_t e = launch_gemm(EPI_FWD, g, st, cnt)) return e;
     }
-    hipLaunchKernelGGL(out_fwd, dim3(nblk(n, 4 * kOutSPW)), dim3(256), 0, st, Z + 3 * L, p, n,
-                       targets, yout, s->dy, s->loss);
+    hipLaunchKernelGGL(out_fwd, dim3(nblk(n, 4 * kOutSPW), cnt), dim3(256), 0, st, Z + 3 * L, sz, p,
+                       n, with_targets ? (const float*)s->tg : nullptr, yout, sy, s->dy, cap,
+                       s->loss);
     return hipGetLastError();
 }
 
-// backward of one member after forward(); gradient accumulated into s->grad (zeroed here)
-hipError_t backward(kwgpu_surrogate* s, int mem, int n, hipStream_t st) {
-    const float* p = s->p[mem];
-    const bf16* wb = s->wb[mem];
+// Backward after forward() for the same members; gradients into s->grad slots 0 .. cnt-1
+// (zeroed here).
+hipError_t backward(kwgpu_surrogate* s, int m0, int cnt, int n, hipStream_t st) {
+    const float* p = s->p[m0];
+    const bf16* wb = s->wb[m0];
+    const long cap = (long)s->cap, sz = 4 * cap * H, sx = cap * F, sdz = cap * H, srow = cap;
     const size_t L = (size_t)n * H;
     float* g = s->grad;
-    hipLaunchKernelGGL(zero_f, dim3(nblk(kP, 256)), dim3(256), 0, st, g, (size_t)kP);
-    const int rs = std::max(1, std::min(64, n / 256));  // row splits of the reductions
-    hipLaunchKernelGGL(out_grad, dim3(H / 64, rs), dim3(256), 0, st, s->Z + 3 * L, s->dy, n, g);
-    hipLaunchKernelGGL(out_bwd, dim3(nblk(L, 256)), dim3(256), 0, st, s->Z + 3 * L, p, s->dy, n,
-                       s->dZa);
+    hipLaunchKernelGGL(zero_f, dim3(nblk((size_t)cnt * kP, 256)), dim3(256), 0, st, g,
+                       (size_t)cnt * kP);
+    // row splits of the column reductions: 16 rows per thread at the default batch
+    const int rs = std::max(1, std::min(128, n / 64));
+    hipLaunchKernelGGL(out_grad, dim3(H / 64, rs, cnt), dim3(256), 0, st, s->Z + 3 * L, sz, s->dy,
+                       srow, n, g);
+    hipLaunchKernelGGL(out_bwd, dim3(nblk(L, 256), cnt), dim3(256), 0, st, s->Z + 3 * L, sz, p,
+                       s->dy, srow, n, s->dZa, sdz);
     bf16* dz = s->dZa;
     bf16* dzn = s->dZb;
     for (int l = 2; l >= 0; --l) {  // hidden layer l+2: weights W_{l+2}, input H_{l+1}
@@ -568,8 +678,12 @@ hipError_t backward(kwgpu_surrogate* s, int mem, int n, hipStream_t st) {
         gw.M = H; gw.N = H; gw.K = n;
         gw.C = g + kW2 + l * (H * H + H);
         gw.k_split = ((n + splits - 1) / splits + 31) / 32 * 32;
-        if (hipError_t e = launch_gemm(EPI_WGRAD, gw, st, splits)) return e;
-        hipLaunchKernelGGL(colsum_bf16, dim3(H / 64, rs), dim3(256), 0, st, dz, n,
+        gw.splits = splits;
+        gw.mA = sdz;
+        gw.mB = sz;
+        gw.mC = kP;
+        if (hipError_t e = launch_gemm(EPI_WGRAD, gw, st, cnt * splits)) return e;
+        hipLaunchKernelGGL(colsum_bf16, dim3(H / 64, rs, cnt), dim3(256), 0, st, dz, sdz, n,
                            g + kB2 + l * (H * H + H));
         GemmArgs gb{};
         gb.A = dz; gb.sam = H; gb.sak = 1;                     // A(m, o) = dZ[m][o]
@@ -577,17 +691,21 @@ hipError_t backward(kwgpu_surrogate* s, int mem, int n, hipStream_t st) {
         gb.M = n; gb.N = H; gb.K = H;
         gb.Z = s->Z + l * L;                                   // pre-activations of H_{l+1}
         gb.Hout = dzn;
-        if (hipError_t e = launch_gemm(EPI_BWD, gb, st)) return e;
+        gb.mA = sdz;
+        gb.mB = 3 * H * H;
+        gb.mZ = sz;
+        gb.mH = sdz;
+        if (hipError_t e = launch_gemm(EPI_BWD, gb, st, cnt)) return e;
         std::swap(dz, dzn);
     }
-    hipLaunchKernelGGL(l1_grad, dim3(H / 64, rs), dim3(256), 0, st, dz, s->x, n, g);
+    hipLaunchKernelGGL(l1_grad, dim3(H / 64, rs, cnt), dim3(256), 0, st, dz, sdz, s->x, sx, n, g);
     return hipGetLastError();
 }
 
-hipError_t refresh_bf16(kwgpu_surrogate* s, int mem, hipStream_t st) {
-    for (int l = 0; l < 3; ++l)
-        hipLaunchKernelGGL(to_bf16, dim3(nblk(H * H, 256)), dim3(256), 0, st,
-                           s->p[mem] + kW2 + l * (H * H + H), s->wb[mem] + l * H * H, H * H);
+// BF16 copies of members m0 .. m0+cnt-1
+hipError_t refresh_bf16(kwgpu_surrogate* s, int m0, int cnt, hipStream_t st) {
+    hipLaunchKernelGGL(to_bf16, dim3(nblk(H * H, 256), 3 * cnt), dim3(256), 0, st, s->p[m0],
+                       s->wb[m0]);
     return hipGetLastError();
 }
 
@@ -633,26 +751,26 @@ int kwgpu_surrogate_create(kwgpu_ctx* ctx, int32_t members, uint64_t seed, kwgpu
     s->device = kwgpu::ctx_device(ctx);
     std::vector<float> w;
     hipStream_t st = kwgpu::ctx_stream(ctx);
+    const size_t M = (size_t)members;
+    if (hipMalloc(&s->pall, M * kP * sizeof(float)) ||
+        hipMalloc(&s->wball, M * 3 * H * H * sizeof(bf16)) ||
+        hipMalloc(&s->grad, M * kP * sizeof(float)) || hipMalloc(&s->m1, M * kP * sizeof(float)) ||
+        hipMalloc(&s->m2, M * kP * sizeof(float)) || hipMalloc(&s->loss, M * sizeof(float))) {
+        kwgpu_surrogate_destroy(s);
+        return kwgpu::ctx_fail(ctx, KWGPU_ERR_NOMEM, "kwgpu_surrogate_create: allocation");
+    }
     for (int m = 0; m < members; ++m) {
-        float* p = nullptr;
-        bf16* b = nullptr;
-        if (hipMalloc(&p, kP * sizeof(float)) || hipMalloc(&b, 3 * H * H * sizeof(bf16))) {
-            kwgpu_surrogate_destroy(s);
-            return kwgpu::ctx_fail(ctx, KWGPU_ERR_NOMEM, "kwgpu_surrogate_create: allocation");
-        }
-        s->p.push_back(p);
-        s->wb.push_back(b);
+        s->p.push_back(s->pall + (size_t)m * kP);
+        s->wb.push_back(s->wball + (size_t)m * 3 * H * H);
         init_params(w, seed * 1000003ull + (uint64_t)m);
-        if (hipMemcpy(p, w.data(), kP * sizeof(float), hipMemcpyHostToDevice) ||
-            refresh_bf16(s, m, st) || hipStreamSynchronize(st)) {
+        if (hipMemcpy(s->p[m], w.data(), kP * sizeof(float), hipMemcpyHostToDevice)) {
             kwgpu_surrogate_destroy(s);
             return kwgpu::ctx_fail(ctx, KWGPU_ERR_HIP, "kwgpu_surrogate_create: upload");
         }
     }
-    if (hipMalloc(&s->grad, kP * sizeof(float)) || hipMalloc(&s->m1, kP * sizeof(float)) ||
-        hipMalloc(&s->m2, kP * sizeof(float)) || hipMalloc(&s->loss, sizeof(float))) {
+    if (refresh_bf16(s, 0, members, st) || hipStreamSynchronize(st)) {
         kwgpu_surrogate_destroy(s);
-        return kwgpu::ctx_fail(ctx, KWGPU_ERR_NOMEM, "kwgpu_surrogate_create: allocation");
+        return kwgpu::ctx_fail(ctx, KWGPU_ERR_HIP, "kwgpu_surrogate_create: upload");
     }
     *out = s;
     return KWGPU_OK;
@@ -662,10 +780,9 @@ void kwgpu_surrogate_destroy(kwgpu_surrogate* s) {
     if (!s) return;
     (void)hipSetDevice(s->device);
     (void)hipDeviceSynchronize();
-    for (auto* p : s->p) (void)hipFree(p);
-    for (auto* b : s->wb) (void)hipFree(b);
     free_scratch(s);
-    for (void* q : {(void*)s->grad, (void*)s->m1, (void*)s->m2, (void*)s->loss, (void*)s->yall})
+    for (void* q : {(void*)s->pall, (void*)s->wball, (void*)s->grad, (void*)s->m1, (void*)s->m2,
+                    (void*)s->loss, (void*)s->yall})
         if (q) (void)hipFree(q);
     delete s;
 }
@@ -716,7 +833,7 @@ int kwgpu_surrogate_set_params(kwgpu_ctx* ctx, kwgpu_surrogate* s, int32_t membe
         return kwgpu::ctx_fail(ctx, KWGPU_ERR_INVALID, "kwgpu_surrogate_set_params: bad argument");
     hipStream_t st = kwgpu::ctx_stream(ctx);
     HIPCHK(hipMemcpy(s->p[member], host, kP * sizeof(float), hipMemcpyHostToDevice));
-    HIPCHK(refresh_bf16(s, member, st));
+    HIPCHK(refresh_bf16(s, member, 1, st));
     HIPCHK(hipStreamSynchronize(st));
     return KWGPU_OK;
 }
@@ -725,15 +842,15 @@ int kwgpu_surrogate_loss_grad(kwgpu_ctx* ctx, kwgpu_surrogate* s, int32_t member
                               const float* host_x, const float* host_t, size_t n, float* host_loss,
                               float* host_grad) {
     if (!s || member < 0 || member >= s->members || !host_x || !host_t || n == 0 ||
-        n > (1u << 20))
+        n > (1u << 17))
         return kwgpu::ctx_fail(ctx, KWGPU_ERR_INVALID, "kwgpu_surrogate_loss_grad: bad argument");
     hipStream_t st = kwgpu::ctx_stream(ctx);
     HIPCHK(ensure_scratch(s, n));
     HIPCHK(hipMemcpy(s->x, host_x, n * F * sizeof(float), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(s->tg, host_t, n * sizeof(float), hipMemcpyHostToDevice));
     hipLaunchKernelGGL(zero_f, dim3(1), dim3(64), 0, st, s->loss, (size_t)1);
-    HIPCHK(forward(s, member, (int)n, s->y, s->tg, st));
-    HIPCHK(backward(s, member, (int)n, st));
+    HIPCHK(forward(s, member, 1, (int)n, false, s->y, (long)s->cap, true, st));
+    HIPCHK(backward(s, member, 1, (int)n, st));
     HIPCHK(hipStreamSynchronize(st));
     float l = 0;
     HIPCHK(hipMemcpy(&l, s->loss, sizeof(float), hipMemcpyDeviceToHost));
@@ -755,7 +872,8 @@ int kwgpu_surrogate_train(kwgpu_ctx* ctx, kwgpu_surrogate* s, const kwgpu_option
     uint32_t* didx = nullptr;
     HIPCHK(hipMalloc(&feat, n * F * sizeof(float)));
     HIPCHK(hipMalloc(&tgt, n * sizeof(float)));
-    HIPCHK(hipMalloc(&didx, n * sizeof(uint32_t)));
+    const int M = s->members;
+    HIPCHK(hipMalloc(&didx, (size_t)M * n * sizeof(uint32_t)));
     int rc = KWGPU_OK;
     do {
         hipLaunchKernelGGL(features_kernel, dim3(nblk(n, 256)), dim3(256), 0, st, d_opts, n,
@@ -764,53 +882,62 @@ int kwgpu_surrogate_train(kwgpu_ctx* ctx, kwgpu_surrogate* s, const kwgpu_option
             rc = kwgpu::ctx_fail(ctx, KWGPU_ERR_NOMEM, std::string("scratch: ") + hipGetErrorString(e));
             break;
         }
-        std::vector<uint32_t> perm(n);
+        // every member steps at once (one launch per layer for the whole ensemble), each
+        // on its own shuffle of the training set
+        std::vector<uint32_t> perm((size_t)M * n);
+        std::vector<std::mt19937_64> rng;
+        for (int m = 0; m < M; ++m) rng.emplace_back(tp->seed + 7919ull * (uint64_t)m);
         const size_t steps_per_epoch = n / B;  // drop the ragged tail
         const double total = (double)steps_per_epoch * tp->epochs;
-        for (int mem = 0; mem < s->members && rc == KWGPU_OK; ++mem) {
-            std::mt19937_64 rng(tp->seed + 7919ull * (uint64_t)mem);
-            hipLaunchKernelGGL(zero_f, dim3(nblk(kP, 256)), dim3(256), 0, st, s->m1, (size_t)kP);
-            hipLaunchKernelGGL(zero_f, dim3(nblk(kP, 256)), dim3(256), 0, st, s->m2, (size_t)kP);
-            long step = 0;
-            for (int ep = 0; ep < tp->epochs && rc == KWGPU_OK; ++ep) {
-                std::iota(perm.begin(), perm.end(), 0u);
-                std::shuffle(perm.begin(), perm.end(), rng);
-                if (hipError_t e = hipMemcpyAsync(didx, perm.data(), n * sizeof(uint32_t),
-                                                  hipMemcpyHostToDevice, st)) {
-                    rc = kwgpu::ctx_fail(ctx, KWGPU_ERR_HIP,
-                                         std::string("upload permutation: ") + hipGetErrorString(e));
-                    break;
-                }
-                hipLaunchKernelGGL(zero_f, dim3(1), dim3(64), 0, st, s->loss, (size_t)1);
-                for (size_t b = 0; b < steps_per_epoch; ++b, ++step) {
-                    hipLaunchKernelGGL(gather_kernel, dim3(nblk(B, 256)), dim3(256), 0, st, feat,
-                                       tgt, didx + b * B, B, s->x, s->tg);
-                    hipError_t e = forward(s, mem, B, s->y, s->tg, st);
-                    if (!e) e = backward(s, mem, B, st);
-                    const double frac = step / total;
-                    const float lr = (float)(tp->lr_final + 0.5 * (tp->lr - tp->lr_final) *
-                                                                 (1.0 + std::cos(M_PI * frac)));
-                    const float c1 = 1.0f - std::pow(tp->beta1, (float)(step + 1));
-                    const float c2 = 1.0f - std::pow(tp->beta2, (float)(step + 1));
-                    hipLaunchKernelGGL(adam_kernel, dim3(nblk(kP, 256)), dim3(256), 0, st,
-                                       s->p[mem], s->grad, s->m1, s->m2, kP, lr, tp->beta1,
-                                       tp->beta2, tp->eps, c1, c2);
-                    if (!e) e = refresh_bf16(s, mem, st);
-                    if (e) {
-                        rc = kwgpu::ctx_fail(ctx, KWGPU_ERR_HIP,
-                                             std::string("train step: ") + hipGetErrorString(e));
-                        break;
-                    }
-                }
-                // the permutation is reused next epoch: wait before overwriting it
-                if (hipError_t e = hipStreamSynchronize(st)) {
-                    rc = kwgpu::ctx_fail(ctx, KWGPU_ERR_HIP, hipGetErrorString(e));
-                    break;
-                }
-                float l = 0;
-                (void)hipMemcpy(&l, s->loss, sizeof(float), hipMemcpyDeviceToHost);
-                if (loss_out) loss_out[(size_t)ep * s->members + mem] = l / (float)(steps_per_epoch * B);
+        const long cap = (long)s->cap;
+        hipLaunchKernelGGL(zero_f, dim3(nblk((size_t)M * kP, 256)), dim3(256), 0, st, s->m1,
+                           (size_t)M * kP);
+        hipLaunchKernelGGL(zero_f, dim3(nblk((size_t)M * kP, 256)), dim3(256), 0, st, s->m2,
+                           (size_t)M * kP);
+        long step = 0;
+        std::vector<float> lsum(M);
+        for (int ep = 0; ep < tp->epochs && rc == KWGPU_OK; ++ep) {
+            for (int m = 0; m < M; ++m) {
+                auto first = perm.begin() + (size_t)m * n;
+                std::iota(first, first + n, 0u);
+                std::shuffle(first, first + n, rng[m]);
             }
+            if (hipError_t e = hipMemcpyAsync(didx, perm.data(), (size_t)M * n * sizeof(uint32_t),
+                                              hipMemcpyHostToDevice, st)) {
+                rc = kwgpu::ctx_fail(ctx, KWGPU_ERR_HIP,
+                                     std::string("upload permutation: ") + hipGetErrorString(e));
+                break;
+            }
+            hipLaunchKernelGGL(zero_f, dim3(1), dim3(64), 0, st, s->loss, (size_t)M);
+            for (size_t b = 0; b < steps_per_epoch; ++b, ++step) {
+                hipLaunchKernelGGL(gather_kernel, dim3(nblk(B, 256), M), dim3(256), 0, st, feat,
+                                   tgt, didx + b * B, (long)n, B, s->x, s->tg, cap);
+                hipError_t e = forward(s, 0, M, B, false, s->y, cap, true, st);
+                if (!e) e = backward(s, 0, M, B, st);
+                const double frac = step / total;
+                const float lr = (float)(tp->lr_final + 0.5 * (tp->lr - tp->lr_final) *
+                                                             (1.0 + std::cos(M_PI * frac)));
+                const float c1 = 1.0f - std::pow(tp->beta1, (float)(step + 1));
+                const float c2 = 1.0f - std::pow(tp->beta2, (float)(step + 1));
+                hipLaunchKernelGGL(adam_kernel, dim3(nblk(kP, 256), M), dim3(256), 0, st, s->pall,
+                                   s->grad, s->m1, s->m2, kP, lr, tp->beta1, tp->beta2, tp->eps,
+                                   c1, c2);
+                if (!e) e = refresh_bf16(s, 0, M, st);
+                if (e) {
+                    rc = kwgpu::ctx_fail(ctx, KWGPU_ERR_HIP,
+                                         std::string("train step: ") + hipGetErrorString(e));
+                    break;
+                }
+            }
+            // the permutation buffer is rewritten next epoch: wait before reusing it
+            if (hipError_t e = hipStreamSynchronize(st)) {
+                rc = kwgpu::ctx_fail(ctx, KWGPU_ERR_HIP, hipGetErrorString(e));
+                break;
+            }
+            (void)hipMemcpy(lsum.data(), s->loss, M * sizeof(float), hipMemcpyDeviceToHost);
+            if (loss_out)
+                for (int m = 0; m < M; ++m)
+                    loss_out[(size_t)ep * M + m] = lsum[m] / (float)(steps_per_epoch * B);
         }
     } while (0);
     (void)hipStreamSynchronize(st);
@@ -827,7 +954,7 @@ int kwgpu_surrogate_predict(kwgpu_ctx* ctx, kwgpu_surrogate* s, const kwgpu_opti
         return kwgpu::ctx_fail(ctx, KWGPU_ERR_INVALID, "kwgpu_surrogate_predict: bad argument");
     if (n == 0) return KWGPU_OK;
     hipStream_t st = stream ? (hipStream_t)stream : kwgpu::ctx_stream(ctx);
-    constexpr size_t CH = 65536;  // samples per pass
+    constexpr size_t CH = 32768;  // samples per pass (scratch: ~1 GB for 4 members)
     HIPCHK(ensure_scratch(s, std::min(n, CH)));
     const size_t need = (size_t)s->members * std::min(n, CH);
     if (s->yall_cap < need) {
@@ -841,8 +968,8 @@ int kwgpu_surrogate_predict(kwgpu_ctx* ctx, kwgpu_surrogate* s, const kwgpu_opti
         hipLaunchKernelGGL(features_kernel, dim3(nblk(cn, 256)), dim3(256), 0, st, d_opts + c0,
                            (size_t)cn, (const double*)nullptr, s->x, (float*)nullptr,
                            d_in_box ? d_in_box + c0 : nullptr);
-        for (int m = 0; m < s->members; ++m)
-            HIPCHK(forward(s, m, cn, s->yall + (size_t)m * cn, nullptr, st));
+        // the whole ensemble at once on the shared features
+        HIPCHK(forward(s, 0, s->members, cn, true, s->yall, cn, false, st));
         hipLaunchKernelGGL(combine_kernel, dim3(nblk(cn, 256)), dim3(256), 0, st, d_opts + c0,
                            s->yall, s->members, (size_t)cn, (size_t)cn, d_premium + c0,
                            d_spread + c0);

@@ -457,4 +457,35 @@ def test_autotune_keeps_results(ctx):
     after = d_p.cpu().numpy()
     assert np.all(d_s.cpu().numpy() == 0)
     assert np.max(np.abs(after - before)) <= EPS
+    assert rep["schedule"]["chosen"] in ("serial", "concurrent")
     c2.close()
+
+
+def test_bucket_schedules_agree(ctx, tmp_path):
+    """An uneven batch's buckets launched one after another on one stream and launched
+    concurrently on the side streams give bitwise the same prices."""
+    import torch
+    _say("bucket schedules")
+    o, xd, td = W.uneven(3000, 79)
+    td = np.minimum(td, 40)
+    d_o, d_x, d_t = _uneven_device(o, xd, td)
+    n = len(o)
+    prm = kwgpu.params(x_dims=d_x.data_ptr(), t_dims=d_t.data_ptr())
+    out = {}
+    for serial in (0, 1):
+        c2 = kwgpu.Context(0)
+        path = str(tmp_path / f"tune{serial}.txt")
+        with open(path, "w") as f:
+            f.write(f"-1 {kwgpu.FP64} {serial} 0\n")
+        c2.tune_load(path)
+        d_p = torch.empty(n, dtype=torch.float64, device="cuda")
+        d_s = torch.empty(n, dtype=torch.int32, device="cuda")
+        for _ in range(2):  # the second call replays the captured graph
+            c2.fd1d_price_device(d_o.data_ptr(), n, d_p.data_ptr(), d_s.data_ptr(), prm)
+            c2.synchronize()
+        assert np.all(d_s.cpu().numpy() == 0)
+        out[serial] = d_p.cpu().numpy().copy()
+        c2.close()
+    assert np.array_equal(out[0], out[1])
+    ref, _ = oracle.fd1d_price_uneven(o[:200], xd[:200], td[:200])
+    assert np.max(np.abs(out[1][:200] - ref)) <= EPS

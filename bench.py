@@ -34,6 +34,7 @@ T_DIM = 1024
 SEED = 1
 FP64_PEAK_TFLOPS = 78.6          # MI355X FP64 (vector and dense matrix)
 HBM_PEAK_GBS = 8000.0
+ERR_BAR = 1e-4                   # max abs price error vs the FP64 FD price for a form to count
 # published FP64 GPU throughput of the reference's own GPU solver at this shape
 # (GTX 1070, batch 32768, 1024 x 1024), options/s
 REFERENCE_GPU_FP64 = 949.0
@@ -54,9 +55,13 @@ def parse_args():
     ap.add_argument("--surrogate", action="store_true",
                     help="also train the premium ensemble on FD labels and time "
                          "surrogate-first serving")
-    ap.add_argument("--sur-train", type=int, default=2_000_000)
+    ap.add_argument("--sur-train", type=int, default=2_000_000,
+                    help="labelled options (FD FP64 1024x1024); 5 %% of them are held out")
     ap.add_argument("--sur-serve", type=int, default=1 << 20)
     ap.add_argument("--sur-epochs", type=int, default=30)
+    ap.add_argument("--next-rows", type=int, default=1,
+                    help="also time FD1D-BS, a compressed chain book and Greeks (host entry "
+                         "points, copies included)")
     ap.add_argument("--peak-probe", type=int, default=1,
                     help="run bin/kwgpu_peak after the timed work (measured FP64 FMA rate "
                          "and copy bandwidth)")
@@ -120,14 +125,18 @@ def cpu_baseline(opts, sample, threads):
     import oracle
     sub = opts[:sample]
     oracle.fd1d_price(sub[:min(16, sample)], X_DIM, T_DIM, nthreads=threads)  # warm-up
-    t0 = time.perf_counter()
-    _, err = oracle.fd1d_price(sub, X_DIM, T_DIM, nthreads=threads)
-    dt = time.perf_counter() - t0
-    if err:
-        raise RuntimeError(err)
+    runs = []
+    for _ in range(3):  # the median of three timed runs
+        t0 = time.perf_counter()
+        _, err = oracle.fd1d_price(sub, X_DIM, T_DIM, nthreads=threads)
+        runs.append(time.perf_counter() - t0)
+        if err:
+            raise RuntimeError(err)
+    dt = float(np.median(runs))
     return {"value": sample / dt, "unit": "options/s", "cores": threads, "kind": "port",
             "sample": f"first {sample} options of the same workload, {X_DIM}x{T_DIM} FP64, "
-                      f"{threads} threads, one job per PDE ({dt:.2f} s); host {cpu_model()}, "
+                      f"{threads} threads, one job per PDE, median of 3 runs "
+                      f"({', '.join(f'{r:.2f}' for r in runs)} s); host {cpu_model()}, "
                       f"{os.cpu_count()} logical CPUs visible"}
 
 
@@ -182,16 +191,73 @@ def bench_uneven(ctx, torch, stream, n, steps, log):
     return out
 
 
+def bench_next_rows(ctx, opts, cpu_threads, log):
+    """The rows beside the solve, through the host-pointer entry points (timings include
+    the host<->device copies): FD1D-BS on the headline workload, a chain-structured
+    book whose PDE compression leaves one solve per 10 options, and Greeks."""
+    import kwgpu
+    import oracle
+    from kwgpu import workloads as W
+
+    def timed(fn, reps):
+        fn()  # warm-up (first call sizes buffers)
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts))
+
+    out = {}
+    n = len(opts)
+    dt = timed(lambda: ctx.fd1d_bs_price(opts, X_DIM, T_DIM), 3)
+    out["fd1d_bs"] = {"workload": f"{n} options (the headline batch), {X_DIM}x{T_DIM}, "
+                                  "American + European FD legs and the BS kernel",
+                      "ms_per_call": dt * 1e3, "options_per_s": n / dt}
+    log(f"bench: FD1D-BS {dt * 1e3:.3f} ms/call")
+
+    base = W.plain(600, 7)
+    book = np.repeat(base, 10)
+    book["s"] = np.tile(np.linspace(80.0, 125.0, 10), 600)
+    book["k"] = np.tile(np.linspace(90.0, 115.0, 10)[::-1], 600)
+    dt = timed(lambda: ctx.fd1d_price(book, 512, 512), 5)
+    out["chain_book"] = {"workload": "6000 options = 600 PDEs x 10 (spot, strike) pairs, "
+                                     "512x512 (the CLI default), seed 7",
+                         "pdes": 600, "ms_per_call": dt * 1e3, "options_per_s": 6000 / dt}
+    if cpu_threads:  # CPU baseline of this row: the oracle compresses the same way
+        t0 = time.perf_counter()
+        _, err = oracle.fd1d_price(book, 512, 512, nthreads=cpu_threads)
+        cpu = time.perf_counter() - t0
+        if err:
+            raise RuntimeError(err)
+        out["chain_book"].update({"cpu_port_ms": cpu * 1e3, "cpu_threads": cpu_threads,
+                                  "speedup_vs_cpu": cpu / dt})
+    log(f"bench: chain book {dt * 1e3:.3f} ms/call")
+
+    sub = opts[:4096]
+    dt = timed(lambda: ctx.fd1d_greeks(sub, X_DIM, T_DIM), 3)
+    out["greeks"] = {"workload": f"first 4096 options of the headline batch, {X_DIM}x{T_DIM}, "
+                                 "price + 8 central bumps of 1 %, spot bumps sharing the PDE",
+                     "ms_per_call": dt * 1e3, "options_per_s": 4096 / dt}
+    log(f"bench: Greeks {dt * 1e3:.3f} ms/call")
+    return out
+
+
 def bench_surrogate(ctx, args, log):
     from kwgpu import pipeline
     from kwgpu import workloads as W
-    sur, train = pipeline.train_and_evaluate(ctx, n_train=args.sur_train,
+    n_test = args.sur_train // 20  # 5 % of the labelled set held out
+    sur, train = pipeline.train_and_evaluate(ctx, n_train=args.sur_train - n_test, n_test=n_test,
                                              epochs=args.sur_epochs, log=log)
     opts = W.serving(args.sur_serve, 4)
     serve = pipeline.serve(ctx, sur, opts, train["spread_bound"], log=log)
+    serve["meets_bar"] = serve["max_abs_err_vs_fp64_fd"] <= pipeline.ERR_BAR
     # the trade-off at looser bars (not the metric: its bar is 1e-4)
-    looser = {bar: pipeline.serve(ctx, sur, opts, v["spread_bound"], repeats=1, log=log)
-              for bar, v in train["looser_bars"].items()}
+    looser = {}
+    for bar, v in train["looser_bars"].items():
+        r = pipeline.serve(ctx, sur, opts, v["spread_bound"], repeats=1, log=log)
+        r["meets_bar"] = r["max_abs_err_vs_fp64_fd"] <= float(bar)
+        looser[bar] = r
     sur.close()
     return {"training": train, "serving": serve, "serving_at_looser_bars": looser}
 
@@ -272,17 +338,28 @@ def main():
         b.copy_(a)
         torch.cuda.synchronize()
 
-    avg_kernel_s = float(np.mean(kernel_ms)) * 1e-3
+    # the metric's value: the fastest form on this workload within the error bar (the
+    # surrogate-first form is measured on its own serving batch and reported beside it)
+    chosen = max((k for k in ("fd_fp64", "fd_fp32_mixed")
+                  if k in forms and forms[k]["max_abs_err_vs_fp64_fd"] <= ERR_BAR),
+                 key=lambda k: forms[k]["options_per_s"])
+    mixed_chosen = chosen == "fd_fp32_mixed"
+    value = forms[chosen]["options_per_s"]
+    ms_value = forms[chosen]["ms_per_step"]
+    avg_kernel_s = float(np.mean(kms_mx if mixed_chosen else kernel_ms)) * 1e-3
     flops = W.fd_flops_per_option(X_DIM, T_DIM) * N_OPTIONS
     achieved = flops / avg_kernel_s / 1e12
     traffic = read_traffic()
-    value = N_OPTIONS / (ms_per_step * 1e-3)
 
     log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
     extra = {}
     if args.uneven > 0 and args.layout != "lane":
         extra["uneven"] = bench_uneven(ctx, torch, stream, args.uneven, max(1, args.steps // 2),
                                        log)
+    if args.next_rows and args.layout != "lane":
+        extra["next_rows"] = bench_next_rows(
+            ctx, opts, min(args.cpu_threads, os.cpu_count() or 1) if args.cpu_sample > 0 else 0,
+            log)
     if args.surrogate:
         extra["surrogate"] = bench_surrogate(ctx, args, log)
         forms["surrogate_first"] = {
@@ -309,30 +386,33 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": ms_value,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": value / REFERENCE_GPU_FP64,
-        "dtype": "f64",
+        "dtype": "f32+f64" if mixed_chosen else "f64",
         "data": "synthetic (seeded SplitMix64 portfolio)",
         "config": {
             "workload": f"plain method: {N_OPTIONS} American options, {X_DIM} x-points x "
                         f"{T_DIM} t-points, FP64, Crank-Nicolson, seed {SEED}",
-            "form": "fd_fp64",
+            "form": chosen,
             "layout": args.layout,
-            "max_abs_err_vs_fp64_fd": 0.0,
+            "max_abs_err_vs_fp64_fd": forms[chosen]["max_abs_err_vs_fp64_fd"],
+            "error_bar": ERR_BAR,
             "forms": forms,
         },
         "roofline": {
             "bound": "mfma",
-            "roof": "FP64 VALU (MI355X FP64 vector peak = FP64 dense peak = 78.6 TFLOP/s)",
+            "roof": "FP64 VALU (MI355X FP64 vector peak = FP64 dense peak = 78.6 TFLOP/s"
+                    + ("; non-packed FP32 issues at the same rate)" if mixed_chosen else ")"),
             "achieved": achieved,
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / FP64_PEAK_TFLOPS,
             "traffic": traffic,
             "kernel": ("fd1d_lane_f64" if args.layout == "lane" else
-                       "fd1d_wave<M=16,L=64,W=1,PERSIST=false,MIXED=false>"),
+                       "fd1d_wave<M=16,L=64,W=1,PERSIST=false,MIXED=%s>"
+                       % ("true" if mixed_chosen else "false")),
             "kernel_ms": avg_kernel_s * 1e3,
             "algorithmic_flops_per_launch": flops,
         },

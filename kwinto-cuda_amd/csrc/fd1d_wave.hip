@@ -116,6 +116,11 @@ struct ScanMul {
                                    // row that takes lane 16 / 32 / 48; L = 32, X16 is
                                    // nonzero in rows 0 and 2 (each segment's first row)
     T FW, ZW;                      // two-wave groups: carries between the waves
+    // two-wave groups, FP64 march with one barrier per step (both waves scan locally,
+    // then exchange one value u): gin += AW u, win += BW u, where u is wave 0's last
+    // forward carry (in wave 1) or wave 1's first backward carry (in wave 0, formed as
+    // z1_local + TW0 c0). AW and BW are per lane and kept in LDS; TW0 is uniform.
+    T TW0;
 };
 
 // Per-PDE state after set-up.
@@ -452,11 +457,9 @@ __device__ __forceinline__ int group_steps(const Pde& d, const Lanes& t, double*
     }
 }
 
-// Carries of one step: forward scan of `c` (the lanes' local last values) giving each
-// lane's f_in; `xchg` links the two waves of a W = 2 group.
-template <int L, int W, class T>
-__device__ __forceinline__ T fwd_carry(T c, const ScanMul<T>& sm, const Lanes& t, bool first_lane,
-                                       T* xchg) {
+// Wave-local inclusive forward scan of the lanes' local last values.
+template <int L, class T>
+__device__ __forceinline__ T fwd_scan(T c, const ScanMul<T>& sm) {
     c = fma(sm.F1, dpp<kRowShr + 1>(c), c);
     c = fma(sm.F2, dpp<kRowShr + 2>(c), c);
     c = fma(sm.F4, dpp<kRowShr + 4>(c), c);
@@ -471,6 +474,36 @@ __device__ __forceinline__ T fwd_carry(T c, const ScanMul<T>& sm, const Lanes& t
         s = dpp<kRowBcast31, 0xC>(c, s);
         c = fma(sm.FB31, s, c);
     }
+    return c;
+}
+
+// Wave-local inclusive backward scan of the lanes' local first values.
+template <int L, class T>
+__device__ __forceinline__ T bwd_scan(T cb, const ScanMul<T>& sm, const Lanes& t) {
+    cb = fma(sm.G1, dpp<kRowShl + 1>(cb), cb);
+    cb = fma(sm.G2, dpp<kRowShl + 2>(cb), cb);
+    cb = fma(sm.G4, dpp<kRowShl + 4>(cb), cb);
+    cb = fma(sm.G8, dpp<kRowShl + 8>(cb), cb);
+    // Across rows. A zero multiplier stands in for a select only inside one option:
+    // 0 * x is NaN when x is not finite, so no lane may read another option's value.
+    if constexpr (L == 64) {
+        cb = fma(sm.X48, readlane<48>(cb), cb);
+        cb = fma(sm.X32, readlane<32>(cb), cb);
+        cb = fma(sm.X16, readlane<16>(cb), cb);
+    } else if constexpr (L == 32) {
+        // two options per wave: each takes the first lane of its own second row
+        const T src = t.wrow < 2 ? readlane<16>(cb) : readlane<48>(cb);
+        cb = fma(sm.X16, src, cb);
+    }
+    return cb;
+}
+
+// Carries of one step: forward scan of `c` (the lanes' local last values) giving each
+// lane's f_in; `xchg` links the two waves of a W = 2 group.
+template <int L, int W, class T>
+__device__ __forceinline__ T fwd_carry(T c, const ScanMul<T>& sm, const Lanes& t, bool first_lane,
+                                       T* xchg) {
+    c = fwd_scan<L>(c, sm);
     T fin;
     if constexpr (W == 2) {
         if (t.wv == 0 && t.lane == 63) xchg[0] = c;
@@ -496,21 +529,7 @@ __device__ __forceinline__ T fwd_carry(T c, const ScanMul<T>& sm, const Lanes& t
 template <int L, int W, class T>
 __device__ __forceinline__ T bwd_carry(T cb, const ScanMul<T>& sm, const Lanes& t, bool last_lane,
                                        T* xchg) {
-    cb = fma(sm.G1, dpp<kRowShl + 1>(cb), cb);
-    cb = fma(sm.G2, dpp<kRowShl + 2>(cb), cb);
-    cb = fma(sm.G4, dpp<kRowShl + 4>(cb), cb);
-    cb = fma(sm.G8, dpp<kRowShl + 8>(cb), cb);
-    // Across rows. A zero multiplier stands in for a select only inside one option:
-    // 0 * x is NaN when x is not finite, so no lane may read another option's value.
-    if constexpr (L == 64) {
-        cb = fma(sm.X48, readlane<48>(cb), cb);
-        cb = fma(sm.X32, readlane<32>(cb), cb);
-        cb = fma(sm.X16, readlane<16>(cb), cb);
-    } else if constexpr (L == 32) {
-        // two options per wave: each takes the first lane of its own second row
-        const T src = t.wrow < 2 ? readlane<16>(cb) : readlane<48>(cb);
-        cb = fma(sm.X16, src, cb);
-    }
+    cb = bwd_scan<L>(cb, sm, t);
     T zin;
     if constexpr (W == 2) {
         if (t.wv == 1 && t.lane == 0) xchg[1] = cb;
@@ -594,6 +613,7 @@ __device__ __forceinline__ void march_f64(double (&xv)[M], const double (&ap)[M]
                                           bool first_lane, bool last_lane, double* xchg,
                                           int wsteps, int steps, double c_th) {
     constexpr int NT = Geo<M, L, W>::NT;
+    const int wv = __builtin_amdgcn_readfirstlane(t.wv);  // wave of a W = 2 group
     for (int step = 0; step < wsteps; ++step) {
         double g[M];
         g[0] = xv[0];
@@ -602,8 +622,29 @@ __device__ __forceinline__ void march_f64(double (&xv)[M], const double (&ap)[M]
         const double c = g[M - 1];
 #pragma unroll
         for (int i = M - 2; i >= 0; --i) g[i] = fma(-up[i], g[i + 1], g[i]);
-        const double gin = fwd_carry<L, W>(c, sm, t, first_lane, xchg);
-        const double win = bwd_carry<L, W>(fma(R[0], gin, g[0]), sm, t, last_lane, xchg);
+        double gin, win;
+        if constexpr (W == 2) {
+            // both waves scan their own rows, then meet once: wave 0 publishes its last
+            // forward carry c0, wave 1 its first backward carry with c0 left out; the
+            // missing c0 terms are linear, with coefficients fixed at set-up. Slots
+            // alternate with the step's parity, so one barrier orders every reuse.
+            const double cw = fwd_scan<L>(c, sm);
+            gin = dpp<kWaveShr1>(cw);
+            const double cb = bwd_scan<L>(fma(R[0], gin, g[0]), sm, t);
+            win = dpp<kWaveShl1>(cb);
+            const int slot = 2 * (step & 1);
+            if (wv == 0 && t.lane == 63) xchg[slot] = cw;
+            if (wv == 1 && t.lane == 0) xchg[slot + 1] = cb;
+            __syncthreads();
+            const double other = xchg[slot + 1 - wv];
+            const double u = wv == 0 ? fma(sm.TW0, readlane<63>(cw), other) : other;
+            const double2 ab = qpv[M * NT + t.gl];
+            gin = fma(ab.x, u, gin);
+            win = fma(ab.y, u, win);
+        } else {
+            gin = fwd_carry<L, W>(c, sm, t, first_lane, xchg);
+            win = bwd_carry<L, W>(fma(R[0], gin, g[0]), sm, t, last_lane, xchg);
+        }
         // one option per workgroup: every step applies (no branch, so the LDS reads and
         // fix-ups can be scheduled into the backward scan)
         if (Geo<M, L, W>::G == 1 || step < steps) {
@@ -634,8 +675,30 @@ __device__ __forceinline__ void solve_group_f64(const FdBatch& b, int group, dou
     const bool last_lane = t.ls == Gm::LT - 1;
     // the same for every lane of the workgroup: keep the loop counter scalar
     const int wsteps = __builtin_amdgcn_readfirstlane(group_steps<M, L, W>(d, t, qp));
+    double aw = 0.0, bw = 0.0;
+    sm.TW0 = 0.0;
+    if constexpr (W == 2) {
+        // wave 1: c0 enters lane l's forward carry with FW of lane l-1 (1 in lane 0) and
+        // its backward carry through the backward scan of R_0 times that; wave 0: wave
+        // 1's first backward carry enters lane l's with ZW of lane l+1 (1 in lane 63)
+        // Every shift runs on the whole wave before any select: a DPP move inside a
+        // branch reads 0 from the lanes the branch turns off.
+        const double fw_prev = dpp<kWaveShr1>(sm.FW);
+        const double zw_next = dpp<kWaveShl1>(sm.ZW);
+        const double fws = t.wv == 1 ? (t.lane == 0 ? 1.0 : fw_prev) : 0.0;
+        const double tw = bwd_scan<L>(R[0] * fws, sm, t);
+        const double tw_next = dpp<kWaveShl1>(tw);
+        const double zws = t.wv == 0 ? (t.lane == 63 ? 1.0 : zw_next) : 0.0;
+        aw = fws;
+        bw = t.wv == 0 ? zws : tw_next;
+        __syncthreads();
+        if (t.wv == 1 && t.lane == 0) xchg[0] = tw;
+        __syncthreads();
+        sm.TW0 = readlane<0>(xchg[0]);  // uniform: keep it in scalar registers
+    }
 
-    // (Q_i, projection target_i) pairs of every thread at qpv[i * NT + gl]
+    // (Q_i, projection target_i) pairs of every thread at qpv[i * NT + gl]; two-wave
+    // groups add (AW, BW) at qpv[M * NT + gl]
     __syncthreads();
     double2* qpv = reinterpret_cast<double2*>(qp);
 #pragma unroll
@@ -643,6 +706,7 @@ __device__ __forceinline__ void solve_group_f64(const FdBatch& b, int group, dou
         const bool proj = d.P.american && d.j0 + i < d.N - 1;
         qpv[i * NT + t.gl] = make_double2(Q[i], proj ? xv[i] : -INFINITY);
     }
+    if constexpr (W == 2) qpv[M * NT + t.gl] = make_double2(aw, bw);
     __syncthreads();
     const int steps = d.steps;
     if (c_th == 1.0)
@@ -758,7 +822,8 @@ template <int M, int L, int W, int WPE, bool PERSIST, bool MIXED>
 __global__ __attribute__((amdgpu_flat_work_group_size(64 * W, 64 * W),
                           amdgpu_waves_per_eu(WPE, WPE)))
 void fd1d_wave(FdBatch b, int* queue, int ngroups) {
-    __shared__ double qp[2 * 64 * W * M];
+    // (Q, payoff) pairs per row and thread, and (AW, BW) per thread for two-wave groups
+    __shared__ double qp[2 * 64 * W * (M + (W == 2 ? 1 : 0))];
     __shared__ double xchg[4];
     auto run = [&](int g) {
         if constexpr (MIXED)

@@ -48,7 +48,6 @@ struct kwgpu_ctx {
     hipStream_t stream = nullptr;
     std::string err;
     DevBuf opts, prices, status, rep, off, list, order, xd, td, ws, counters;
-    DevBuf opts2, prices2, status2;  // second leg of FD1D-BS
     // autotuned choices per (bucket, precision), and per precision whether an uneven
     // batch's buckets run one after another on the caller's stream (true) or
     // concurrently on the side streams (false, the default)
@@ -60,6 +59,7 @@ struct kwgpu_ctx {
     bool plan_valid = false;
     uint64_t plan_key = 0;              // hash of everything but the sizes
     std::vector<int32_t> plan_x, plan_t;  // the sizes the plan was made for (compared exactly)
+    int plan_prec = -1, plan_layout = -1;  // ... and the precision and layout it bucketed for
     int32_t* hsizes = nullptr;          // pinned staging for device-resident sizes
     size_t hsizes_cap = 0;
     std::vector<Bucket> plan;
@@ -158,6 +158,26 @@ uint64_t fnv(uint64_t h, const void* p, size_t n) {
     return h;
 }
 
+// Hashes of the fields themselves: the structs hold padding whose bytes are unspecified
+// after aggregate initialisation or a copy, so hashing their storage would change the
+// key from call to call.
+template <class... T>
+uint64_t fnv_fields(uint64_t h, const T&... v) {
+    ((h = fnv(h, &v, sizeof(v))), ...);
+    return h;
+}
+
+uint64_t fnv_batch(uint64_t h, const FdBatch& b) {
+    return fnv_fields(h, b.opts, b.rep, b.off, b.list, b.order, b.x_dims, b.t_dims, b.m, b.x_dim,
+                      b.t_dim, b.density, b.scale, b.theta, b.prices, b.status, b.ws, b.ws_stride,
+                      b.ws_rows);
+}
+
+uint64_t fnv_bucket(uint64_t h, const Bucket& k) {
+    const int persist = k.ch.persist ? 1 : 0;
+    return fnv_fields(h, k.cap, k.ch.variant, persist, k.begin, k.count, k.max_x);
+}
+
 int lane_launch(kwgpu_ctx* c, FdBatch bb, int max_x, int precision, hipStream_t s, double* ws) {
     if (precision != KWGPU_FP64)
         return fail(c, KWGPU_ERR_UNSUPPORTED, "kwgpu: FP32-mixed runs in the wave layout only");
@@ -239,7 +259,16 @@ int run_fd(kwgpu_ctx* c, FdBatch b, const int32_t* pde_x, const int32_t* pde_t, 
     const bool uniform = pde_x == nullptr;
     if (precision == KWGPU_FP32_MIXED && layout == KWGPU_LAYOUT_LANE_SOA)
         return fail(c, KWGPU_ERR_UNSUPPORTED, "kwgpu: FP32-mixed runs in the wave layout only");
-    const int mx = uniform ? b.x_dim : *std::max_element(pde_x, pde_x + b.m);
+    // a batch with exactly the cached plan's sizes, precision and layout has the plan's
+    // buckets: checked first, so a replay does no per-PDE pass but these two compares
+    const size_t bytes = sizeof(int32_t) * (size_t)b.m;
+    const bool same_sizes = !uniform && c->plan_valid && c->plan_prec == precision &&
+                            c->plan_layout == layout && c->plan_x.size() == (size_t)b.m &&
+                            !std::memcmp(c->plan_x.data(), pde_x, bytes) &&
+                            !std::memcmp(c->plan_t.data(), pde_t, bytes);
+    const int mx = uniform      ? b.x_dim
+                   : same_sizes ? c->plan.back().max_x
+                                : *std::max_element(pde_x, pde_x + b.m);
     if (layout == KWGPU_LAYOUT_WAVE_REG && !kwgpu::wave_layout_fits(mx, precision))
         return fail(c, KWGPU_ERR_UNSUPPORTED, "kwgpu: wave layout supports x_dim <= 2048");
     if (precision == KWGPU_FP32_MIXED && !kwgpu::wave_layout_fits(mx, precision))
@@ -269,30 +298,35 @@ int run_fd(kwgpu_ctx* c, FdBatch b, const int32_t* pde_x, const int32_t* pde_t, 
         const int cap = bucket_of(x, precision, layout);
         return cap == kLaneCap ? kSlots - 1 : 31 - __builtin_clz((unsigned)cap) - 6;
     };
-    size_t count[kSlots] = {};
-    int widest[kSlots] = {};
-    for (int p = 0; p < m; ++p) {
-        const int k = slot_of(pde_x[p]);
-        ++count[k];
-        widest[k] = std::max(widest[k], pde_x[p]);
-    }
     std::vector<Bucket> plan;
-    size_t begin = 0;
-    for (int k = 0; k < kSlots; ++k) {
-        if (!count[k]) continue;
-        const int cap = k == kSlots - 1 ? kLaneCap : 64 << k;
-        plan.push_back(Bucket{cap, choice_for(c, cap, precision, layout), begin, count[k], widest[k]});
-        begin += count[k];
+    if (same_sizes) {
+        plan = c->plan;
+        for (Bucket& k : plan) k.ch = choice_for(c, k.cap, precision, layout);
+    } else {
+        size_t count[kSlots] = {};
+        int widest[kSlots] = {};
+        for (int p = 0; p < m; ++p) {
+            const int k = slot_of(pde_x[p]);
+            ++count[k];
+            widest[k] = std::max(widest[k], pde_x[p]);
+        }
+        size_t begin = 0;
+        for (int k = 0; k < kSlots; ++k) {
+            if (!count[k]) continue;
+            const int cap = k == kSlots - 1 ? kLaneCap : 64 << k;
+            plan.push_back(Bucket{cap, choice_for(c, cap, precision, layout), begin, count[k], widest[k]});
+            begin += count[k];
+        }
     }
     // the batch's identity: buffers, parameters and kernel choices (hashed) and the sizes
     // (compared exactly with the plan's own copy)
     uint64_t key = 1469598103934665603ull;
-    key = fnv(key, &b, sizeof(b));
-    key = fnv(key, &precision, sizeof(precision));
-    for (const Bucket& k : plan) key = fnv(key, &k, sizeof(k));
+    key = fnv_batch(key, b);
+    key = fnv_fields(key, precision);
+    for (const Bucket& k : plan) key = fnv_bucket(key, k);
     const auto sit = c->serial.find(precision);
-    const bool serial = sit != c->serial.end() && sit->second;
-    key = fnv(key, &serial, sizeof(serial));
+    const int serial = sit != c->serial.end() && sit->second ? 1 : 0;
+    key = fnv_fields(key, serial);
 
     // ... and the context buffers a captured graph would reference
     auto with_buffers = [&](uint64_t k) {
@@ -300,10 +334,7 @@ int run_fd(kwgpu_ctx* c, FdBatch b, const int32_t* pde_x, const int32_t* pde_t, 
         return fnv(k, bufs, sizeof(bufs));
     };
     if (int rc = ensure_side_streams(c)) return rc;
-    const size_t bytes = sizeof(int32_t) * (size_t)m;
-    const bool rebuild = !c->plan_valid || c->plan_key != with_buffers(key) ||
-                         c->plan_x.size() != (size_t)m || std::memcmp(c->plan_x.data(), pde_x, bytes) ||
-                         std::memcmp(c->plan_t.data(), pde_t, bytes);
+    const bool rebuild = !same_sizes || c->plan_key != with_buffers(key);
     if (rebuild) {
         drop_graph(c);
         size_t ws = 0;
@@ -330,6 +361,8 @@ int run_fd(kwgpu_ctx* c, FdBatch b, const int32_t* pde_x, const int32_t* pde_t, 
         c->plan_key = key;
         c->plan_x.assign(pde_x, pde_x + m);
         c->plan_t.assign(pde_t, pde_t + m);
+        c->plan_prec = precision;
+        c->plan_layout = layout;
         c->plan_valid = true;
     }
     if (!c->use_graphs) return launch_buckets(c, b, c->plan, precision, s, serial);
@@ -359,6 +392,74 @@ int run_fd(kwgpu_ctx* c, FdBatch b, const int32_t* pde_x, const int32_t* pde_t, 
     return KWGPU_OK;
 }
 
+// PDE compression (src/Pricer/kwFd1d.cpp:28-65): options equal in (t, r, q, z, e, w) and
+// grid size share one solve. The reference sorts the options by that key; only the
+// grouping matters here (each PDE is solved from its first option's fields), so the key
+// goes into an open-addressing hash table, O(n), and a counting pass lays out each PDE's
+// options (rep = first option of each PDE, off = CSR offsets into list). Fields compare
+// with ==, so a NaN field never matches and -0.0 matches +0.0, as under the reference's
+// `<`; options whose key holds a NaN each get a PDE of their own (they fail as bad
+// input either way).
+void group_pdes(const kwgpu_option* opts, size_t n, const int32_t* xd, const int32_t* td, int x_dim,
+                int t_dim, std::vector<uint32_t>& rep, std::vector<uint32_t>& off,
+                std::vector<uint32_t>& list) {
+    auto gx = [&](size_t i) { return xd ? xd[i] : x_dim; };
+    auto gt = [&](size_t i) { return td ? td[i] : t_dim; };
+    auto same = [&](size_t i, size_t j) {
+        const kwgpu_option &a = opts[i], &b = opts[j];
+        return a.t == b.t && a.r == b.r && a.q == b.q && a.z == b.z && a.e == b.e && a.w == b.w &&
+               gx(i) == gx(j) && gt(i) == gt(j);
+    };
+    auto mix = [](uint64_t h, double v) {
+        uint64_t u;
+        v = v == 0.0 ? 0.0 : v;  // -0.0 hashes as +0.0
+        std::memcpy(&u, &v, sizeof(u));
+        h ^= u + 0x9e3779b97f4a7c15ull + (h << 6) + (h >> 2);
+        return h;
+    };
+    size_t cap = 16;
+    while (cap < 2 * n) cap <<= 1;
+    std::vector<int32_t> table(cap, -1);  // PDE id per slot
+    std::vector<uint32_t> pde(n), count;
+    rep.clear();
+    for (size_t i = 0; i < n; ++i) {
+        const kwgpu_option& a = opts[i];
+        uint64_t h = 0;
+        h = mix(h, a.t);
+        h = mix(h, a.r);
+        h = mix(h, a.q);
+        h = mix(h, a.z);
+        h = mix(h, (double)(a.e * 4 + a.w));
+        h = mix(h, (double)gx(i) * 65536.0 + (double)gt(i));
+        h ^= h >> 31;
+        h *= 0xbf58476d1ce4e5b9ull;
+        h ^= h >> 29;
+        size_t slot = h & (cap - 1);
+        for (;;) {
+            const int32_t id = table[slot];
+            if (id < 0) {
+                table[slot] = (int32_t)rep.size();
+                pde[i] = (uint32_t)rep.size();
+                rep.push_back((uint32_t)i);
+                count.push_back(1);
+                break;
+            }
+            if (same(rep[id], i)) {
+                pde[i] = (uint32_t)id;
+                ++count[id];
+                break;
+            }
+            slot = (slot + 1) & (cap - 1);
+        }
+    }
+    const size_t m = rep.size();
+    off.assign(m + 1, 0);
+    for (size_t k = 0; k < m; ++k) off[k + 1] = off[k] + count[k];
+    list.resize(n);
+    std::vector<uint32_t> fill(off.begin(), off.end() - 1);
+    for (size_t i = 0; i < n; ++i) list[fill[pde[i]]++] = (uint32_t)i;  // ascending per PDE
+}
+
 // Host-pointer FD price with the reference's PDE compression; results are left in
 // the device buffers `d_prices` / `d_status` (sized n). Returns KWGPU_OK or an error.
 int fd_price_host(kwgpu_ctx* c, const kwgpu_option* opts, size_t n, const kwgpu_fd1d_params* p,
@@ -368,22 +469,8 @@ int fd_price_host(kwgpu_ctx* c, const kwgpu_option* opts, size_t n, const kwgpu_
     // -- and, for uneven batches, in grid size -- share one solve.
     auto gx = [&](size_t i) { return p->x_dims ? p->x_dims[i] : p->x_dim; };
     auto gt = [&](size_t i) { return p->t_dims ? p->t_dims[i] : p->t_dim; };
-    auto key = [&](size_t i) {
-        const auto& a = opts[i];
-        return std::make_tuple(a.t, a.r, a.q, a.z, a.e, a.w, gx(i), gt(i));
-    };
-    std::vector<uint32_t> sorted(n);
-    std::iota(sorted.begin(), sorted.end(), 0u);
-    std::stable_sort(sorted.begin(), sorted.end(),
-                     [&](uint32_t l, uint32_t r) { return key(l) < key(r); });
-    std::vector<uint32_t> rep, off, list(sorted);
-    for (size_t i = 0; i < n; ++i) {
-        if (i == 0 || key(sorted[i - 1]) < key(sorted[i])) {
-            rep.push_back(sorted[i]);
-            off.push_back((uint32_t)i);
-        }
-    }
-    off.push_back((uint32_t)n);
+    std::vector<uint32_t> rep, off, list;
+    group_pdes(opts, n, p->x_dims, p->t_dims, p->x_dim, p->t_dim, rep, off, list);
     const size_t m = rep.size();
 
     hipError_t e;
@@ -536,7 +623,7 @@ void kwgpu_destroy(kwgpu_ctx* c) {
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
     if (c->hsizes) (void)hipHostFree(c->hsizes);
     for (DevBuf* b : {&c->opts, &c->prices, &c->status, &c->rep, &c->off, &c->list, &c->order,
-                      &c->xd, &c->td, &c->ws, &c->counters, &c->opts2, &c->prices2, &c->status2})
+                      &c->xd, &c->td, &c->ws, &c->counters})
         free_buf(*b);
     (void)hipStreamDestroy(c->stream);
     delete c;
@@ -667,28 +754,40 @@ int kwgpu_fd1d_bs_price(kwgpu_ctx* c, const kwgpu_option* opts, size_t n,
     if (int rc = check_dims(c, p->x_dims, p->t_dims, n)) return rc;
     if (hipError_t e = hipSetDevice(c->device); e != hipSuccess) return hip_fail(c, e, "hipSetDevice");
     const char* prefix = "Fd1d_BlackScholes_Pricer::price : ";
-    // leg 1: FD on the options as given (src/Pricer/kwFd1d_BlackScholes.cpp:18)
-    if (int rc = fd_price_host(c, opts, n, p, c->opts, c->prices, c->status)) return rc;
-    std::vector<int32_t> st(n), st2(n);
+    // legs 1 and 2 in one solve: the options as given (src/Pricer/kwFd1d_BlackScholes.cpp:18)
+    // followed by their European copies (:21-28), 2n options on the same grids
+    std::vector<kwgpu_option> both(2 * n);
+    std::copy(opts, opts + n, both.begin());
+    std::copy(opts, opts + n, both.begin() + n);
+    for (size_t i = n; i < 2 * n; ++i) both[i].e = 0;
+    kwgpu_fd1d_params p2 = *p;
+    std::vector<int32_t> xd2, td2;
+    if (p->x_dims) {
+        xd2.assign(p->x_dims, p->x_dims + n);
+        xd2.insert(xd2.end(), p->x_dims, p->x_dims + n);
+        p2.x_dims = xd2.data();
+    }
+    if (p->t_dims) {
+        td2.assign(p->t_dims, p->t_dims + n);
+        td2.insert(td2.end(), p->t_dims, p->t_dims + n);
+        p2.t_dims = td2.data();
+    }
+    if (int rc = fd_price_host(c, both.data(), 2 * n, &p2, c->opts, c->prices, c->status)) return rc;
     hipError_t e;
-    if ((e = hipMemcpy(st.data(), c->status.p, n * sizeof(int32_t), hipMemcpyDeviceToHost)))
-        return hip_fail(c, e, "kwgpu: download");
-    // leg 2: FD on European copies (:21-28)
-    std::vector<kwgpu_option> euro(opts, opts + n);
-    for (auto& a : euro) a.e = 0;
-    if (int rc = fd_price_host(c, euro.data(), n, p, c->opts2, c->prices2, c->status2)) return rc;
-    if ((e = hipMemcpy(st2.data(), c->status2.p, n * sizeof(int32_t), hipMemcpyDeviceToHost)))
-        return hip_fail(c, e, "kwgpu: download");
     // leg 3 and combine: price += BS(euro) - FD(euro) (:30-40)
     if ((e = kwgpu::launch_fd1d_bs_combine((const kwgpu_option*)c->opts.p, n, (double*)c->prices.p,
-                                           (const double*)c->prices2.p, c->stream)))
+                                           (const double*)c->prices.p + n, c->stream)))
         return hip_fail(c, e, "kwgpu: combine kernel launch");
+    std::vector<int32_t> st(2 * n);
     if ((e = hipMemcpyAsync(prices, c->prices.p, n * sizeof(double), hipMemcpyDeviceToHost,
+                            c->stream)) ||
+        (e = hipMemcpyAsync(st.data(), c->status.p, 2 * n * sizeof(int32_t), hipMemcpyDeviceToHost,
                             c->stream)) ||
         (e = hipStreamSynchronize(c->stream)))
         return hip_fail(c, e, "kwgpu: download");
     for (size_t i = 0; i < n; ++i)
-        if (st[i] == KWGPU_STATUS_OK && st2[i] != KWGPU_STATUS_OK) st[i] = st2[i];
+        if (st[i] == KWGPU_STATUS_OK && st[n + i] != KWGPU_STATUS_OK) st[i] = st[n + i];
+    st.resize(n);
     if (status) std::memcpy(status, st.data(), n * sizeof(int32_t));
     return collect(c, opts, n, p, st, prefix);
 }
@@ -832,6 +931,7 @@ int kwgpu_fd1d_autotune(kwgpu_ctx* c, const kwgpu_option* d_opts, size_t n,
         struct Cand {
             Choice ch;
             float ms;
+            bool pruned = false;  // ms is a lower bound; the candidate was not run
         };
         std::vector<Cand> cands;
         for (int v = 0; v < kwgpu::wave_variant_count(); ++v)
@@ -843,7 +943,44 @@ int kwgpu_fd1d_autotune(kwgpu_ctx* c, const kwgpu_option* d_opts, size_t n,
         constexpr size_t kLaneSample = 256;
         FdBatch bl = bb;
         bl.m = (int)std::min(S, kLaneSample);
+        // the lane layout's lower bound: one option runs as one thread's serial chain, so
+        // the bucket takes at least (the longest option's point-steps) x (the time per
+        // point-step of a single option run alone, on the cheapest option of the sample)
+        size_t kmin = 0;
+        double pts_min = 0.0, pts_max = 0.0;
+        for (size_t k = 0; k < S; ++k) {
+            const double pts = (double)hx[smp[k]] * (ht[smp[k]] - 1);
+            if (k == 0 || pts < pts_min) pts_min = pts, kmin = k;
+            pts_max = std::max(pts_max, pts);
+        }
         for (auto& cd : cands) {
+            if (cd.ch.variant < 0 && pts_min > 0.0) {
+                float best_wave = 0.f;
+                for (const auto& w : cands)
+                    if (w.ch.variant >= 0 && (best_wave == 0.f || w.ms < best_wave)) best_wave = w.ms;
+                FdBatch bp = bb;
+                bp.m = 1;
+                bp.order = (const uint32_t*)sample.p + kmin;
+                if (ensure(ws, kwgpu::lane_workspace_doubles(1, bx) * sizeof(double))) {
+                    rc = fail(c, KWGPU_ERR_NOMEM, "kwgpu_fd1d_autotune: workspace");
+                    break;
+                }
+                float pilot = 0.f;
+                for (int r = 0; r < 2 && !rc; ++r) {  // the first run loads the code object
+                    if ((e = hipEventRecord(e0, s))) { rc = hip_fail(c, e, "event"); break; }
+                    if ((rc = lane_launch(c, bp, bx, prec, s, (double*)ws.p))) break;
+                    if ((e = hipEventRecord(e1, s)) || (e = hipEventSynchronize(e1)) ||
+                        (e = hipEventElapsedTime(&pilot, e0, e1)))
+                        rc = hip_fail(c, e, "event");
+                }
+                if (rc) break;
+                const float bound = (float)(pilot / pts_min * pts_max);
+                if (best_wave > 0.f && bound >= best_wave) {
+                    cd.ms = bound;
+                    cd.pruned = true;
+                    continue;
+                }
+            }
             auto run = [&]() -> int {
                 if (cd.ch.variant < 0) {
                     if (ensure(ws, kwgpu::lane_workspace_doubles(bl.m, bx) * sizeof(double)))
@@ -886,7 +1023,8 @@ int kwgpu_fd1d_autotune(kwgpu_ctx* c, const kwgpu_option* d_opts, size_t n,
         for (size_t k = 0; k < cands.size(); ++k) {
             const auto& cd = cands[k];
             if (cd.ch.variant < 0) {
-                std::snprintf(buf, sizeof(buf), "%s{\"layout\": \"lane\", \"ms\": %.4f}", k ? ", " : "", cd.ms);
+                std::snprintf(buf, sizeof(buf), "%s{\"layout\": \"lane\", \"ms\": %.4f, \"pruned\": %s}",
+                              k ? ", " : "", cd.ms, cd.pruned ? "true" : "false");
             } else {
                 const auto wv = kwgpu::wave_variant(cd.ch.variant);
                 std::snprintf(buf, sizeof(buf),

@@ -35,6 +35,13 @@ constexpr int kW1 = 0, kB1 = kW1 + H * F, kW2 = kB1 + H, kB2 = kW2 + H * H, kW3 
               kB3 = kW3 + H * H, kW4 = kB3 + H, kB4 = kW4 + H * H, kW5 = kB4 + H, kB5 = kW5 + H,
               kP = kB5 + 1;
 static_assert(kP == KWGPU_SURROGATE_PARAMS, "parameter layout");
+// partial-sum slots of the column reductions: at most kPartRows row blocks, each
+// at most kPartWidth sums (the layer-1 block W1 | b1), per member
+constexpr int kPartRows = 128, kPartWidth = H * F + H;
+// ... and of the weight-gradient GEMMs' K splits (at most kMaxSplits slots of H x H)
+constexpr int kMaxSplits = 32;
+constexpr size_t kPartFloats = std::max<size_t>((size_t)kPartRows * kPartWidth,
+                                                (size_t)kMaxSplits * H * H);
 // Target: y = (premium - baw) / (baw + kEps K), the part of the early-exercise premium
 // that the quadratic approximation below (baw) misses, relative to that approximation
 // (floored at kEps K). A network error dy then costs (baw + kEps K) dy in price: small
@@ -186,10 +193,11 @@ struct GemmArgs {
     const float* bias;  // EPI_FWD: bias[n]
     float* Z;           // EPI_FWD: writes pre-activations; EPI_BWD: reads them
     bf16* Hout;         // EPI_FWD: silu(Z); EPI_BWD: dZ = C * silu'(Z)
-    float* C;           // EPI_WGRAD: C[m * N + n] += sum over this block's K range
+    float* C;           // EPI_WGRAD: C[z][m * N + n] = sum over this block's K range, one
+                        // M x N slot per blockIdx.z (member, split), summed afterwards
     int k_split;        // EPI_WGRAD: K range per split
     int splits;         // EPI_WGRAD: K splits; blockIdx.z = member * splits + split
-    long mA, mB, mBias, mZ, mH, mC;  // member strides (elements)
+    long mA, mB, mBias, mZ, mH;  // member strides (elements)
 };
 
 // C(M x N) = A(M x K) B(K x N), 64 x 64 tile per 256-thread workgroup, each wave a
@@ -214,7 +222,7 @@ __global__ __launch_bounds__(256) void gemm_bf16(GemmArgs g) {
         g.Z += mem * g.mZ;
         g.Hout += mem * g.mH;
     } else {
-        g.C += mem * g.mC;
+        g.C += (long)blockIdx.z * g.M * g.N;
     }
     const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
     floatx4 acc[2][2];
@@ -307,7 +315,7 @@ __global__ __launch_bounds__(256) void gemm_bf16(GemmArgs g) {
                 } else if constexpr (EPI == EPI_BWD) {
                     g.Hout[o] = (bf16)(v * dsilu(g.Z[o]));
                 } else {
-                    atomicAdd(&g.C[o], v);
+                    g.C[o] = v;
                 }
             }
 }
@@ -381,24 +389,24 @@ __device__ __forceinline__ float phase_sum(float v, float* lds) {
     const int c = threadIdx.x & 63;
     return lds[c] + lds[64 + c] + lds[128 + c] + lds[192 + c];
 }
-__global__ __launch_bounds__(256) void colsum_bf16(const bf16* dZ, long sdz, int n, float* out) {
+__global__ __launch_bounds__(256) void colsum_bf16(const bf16* dZ, long sdz, int n, float* part) {
     __shared__ float lds[256];
     dZ += blockIdx.z * sdz;
-    out += (long)blockIdx.z * kP;
+    part += ((long)blockIdx.z * gridDim.y + blockIdx.y) * H;  // this block's column sums
     const int col = blockIdx.x * 64 + (threadIdx.x & 63);
     const int r0 = blockIdx.y * 4 + (threadIdx.x >> 6);
     float s = 0.0f;
 #pragma unroll 4
     for (int m = r0; m < n; m += 4 * gridDim.y) s += (float)dZ[(long)m * H + col];
     s = phase_sum(s, lds);
-    if (threadIdx.x < 64) atomicAdd(&out[col], s);
+    if (threadIdx.x < 64) part[col] = s;
 }
 __global__ __launch_bounds__(256) void out_grad(const float* Z4, long sz, const float* dy,
-                                                long srow, int n, float* grad) {
+                                                long srow, int n, float* part) {
     __shared__ float lds[256];
     Z4 += blockIdx.z * sz;
     dy += blockIdx.z * srow;
-    grad += (long)blockIdx.z * kP;
+    part += ((long)blockIdx.z * gridDim.y + blockIdx.y) * (H + 1);  // this block's W5 | b5 sums
     const int col = blockIdx.x * 64 + (threadIdx.x & 63);
     const int r0 = blockIdx.y * 4 + (threadIdx.x >> 6);
     float s = 0.0f, sb = 0.0f;
@@ -410,32 +418,90 @@ __global__ __launch_bounds__(256) void out_grad(const float* Z4, long sz, const 
     __syncthreads();
     sb = phase_sum(sb, lds);
     if (threadIdx.x < 64) {
-        atomicAdd(&grad[kW5 + col], s);
-        if (col == 0) atomicAdd(&grad[kB5], sb);
+        part[col] = s;
+        if (col == 0) part[H] = sb;
     }
 }
 // input layer gradients: grad W1[j][f] += sum_m dZ1[m][j] x[m][f], b1[j] += sum_m dZ1[m][j]
+// Layer-1 gradient: seven sums per hidden column (six W1 weights and the bias). Each
+// lane owns four adjacent columns, so a wave reads one whole 512-byte dZ1 row per
+// iteration and the row's six inputs once (wave-uniform); the four waves' partial sums
+// meet in LDS and each block adds 1792 values with one atomic each.
+typedef bf16 bf16x4 __attribute__((ext_vector_type(4)));
 __global__ __launch_bounds__(256) void l1_grad(const bf16* dZ1, long sdz, const float* x, long sx,
-                                               int n, float* grad) {
+                                               int n, float* part) {
+    static_assert(H == 256, "one wave covers a row as 64 lanes x 4 columns");
+    constexpr int V = 4 * (F + 1);
+    __shared__ float lds[V * 256];
+    dZ1 += blockIdx.y * sdz;
+    x += blockIdx.y * sx;
+    part += ((long)blockIdx.y * gridDim.x + blockIdx.x) * (H * F + H);  // W1 | b1 order
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    float s[V] = {};
+    // U rows per wave per pass, all loads issued before any use (one wave per SIMD
+    // here, so the loads' latency is hidden only by having several in flight)
+    constexpr int U = 8;
+    const int stride = 4 * gridDim.x;
+    for (int m0 = blockIdx.x * 4 + w; m0 < n; m0 += U * stride) {
+        bf16x4 d4[U];
+        float xf[U][F];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            // rows past the end load the last row (no branch around the loads) and
+            // contribute d = 0
+            const int m = m0 + u * stride;
+            const int mc = min(m, n - 1);
+            const bf16x4 v = *(const bf16x4*)(dZ1 + (long)mc * H + 4 * lane);
+            d4[u] = m < n ? v : bf16x4{};
+#pragma unroll
+            for (int f = 0; f < F; ++f) xf[u][f] = x[mc * F + f];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const float d = (float)d4[u][j];
+#pragma unroll
+                for (int f = 0; f < F; ++f) s[j * (F + 1) + f] = fmaf(d, xf[u][f], s[j * (F + 1) + f]);
+                s[j * (F + 1) + F] += d;
+            }
+    }
+#pragma unroll
+    for (int v = 0; v < V; ++v) lds[v * 256 + threadIdx.x] = s[v];
+    __syncthreads();
+    for (int o = threadIdx.x; o < V * 64; o += 256) {
+        const int v = o >> 6, l = o & 63;
+        const float t = lds[v * 256 + l] + lds[v * 256 + 64 + l] + lds[v * 256 + 128 + l] +
+                        lds[v * 256 + 192 + l];
+        const int col = 4 * l + v / (F + 1), f = v % (F + 1);
+        part[f < F ? col * F + f : H * F + col] = t;
+    }
+}
+
+// grad[member][offset + o] += sum over the nb blocks' partial sums part[member][b][o]: the
+// second stage of the column reductions above. Many blocks adding into the same few
+// hundred addresses with device-scope atomics serialise on them; this reads each
+// partial once, coalesced.
+// 64 outputs per 256-thread block: the four waves take every fourth partial, 4 loads in
+// flight each, and meet in LDS.
+__global__ __launch_bounds__(256) void sum_parts(const float* part, int nb, int width,
+                                                 float* grad, int offset) {
     __shared__ float lds[256];
-    dZ1 += blockIdx.z * sdz;
-    x += blockIdx.z * sx;
-    grad += (long)blockIdx.z * kP;
-    const int col = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int r0 = blockIdx.y * 4 + (threadIdx.x >> 6);
-    float s[F + 1] = {};
-    for (int m = r0; m < n; m += 4 * gridDim.y) {
-        const float d = (float)dZ1[(long)m * H + col];
-#pragma unroll
-        for (int f = 0; f < F; ++f) s[f] = fmaf(d, x[m * F + f], s[f]);
-        s[F] += d;
+    const int o = blockIdx.x * 64 + (threadIdx.x & 63), ph = threadIdx.x >> 6;
+    const int oc = min(o, width - 1);  // lanes past the end load a valid slot, write nothing
+    part += (long)blockIdx.y * nb * width + oc;
+    float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, s3 = 0.0f;
+    int b = ph;
+    for (; b + 12 < nb; b += 16) {
+        s0 += part[(long)b * width];
+        s1 += part[(long)(b + 4) * width];
+        s2 += part[(long)(b + 8) * width];
+        s3 += part[(long)(b + 12) * width];
     }
-#pragma unroll
-    for (int f = 0; f <= F; ++f) {
-        const float t = phase_sum(s[f], lds);
-        if (threadIdx.x < 64) atomicAdd(&grad[f < F ? kW1 + col * F + f : kB1 + col], t);
-        __syncthreads();
-    }
+    for (; b < nb; b += 4) s0 += part[(long)b * width];
+    const float t = phase_sum((s0 + s1) + (s2 + s3), lds);
+    if (threadIdx.x < 64 && o < width) grad[(long)blockIdx.y * kP + offset + o] += t;
 }
 
 __global__ void adam_kernel(float* p, const float* g, float* m1, float* m2, int np, float lr,
@@ -559,6 +625,8 @@ struct kwgpu_surrogate {
     uint32_t* idx = nullptr;
     float* yall = nullptr;
     size_t yall_cap = 0;
+    float* part = nullptr;  // per-block partial sums of the column reductions (kPartRows
+                            // blocks x kPartWidth per member)
 };
 
 namespace {
@@ -663,28 +731,33 @@ hipError_t backward(kwgpu_surrogate* s, int m0, int cnt, int n, hipStream_t st) 
     hipLaunchKernelGGL(zero_f, dim3(nblk((size_t)cnt * kP, 256)), dim3(256), 0, st, g,
                        (size_t)cnt * kP);
     // row splits of the column reductions: 16 rows per thread at the default batch
-    const int rs = std::max(1, std::min(128, n / 64));
+    const int rs = std::max(1, std::min(kPartRows, n / 64));
     hipLaunchKernelGGL(out_grad, dim3(H / 64, rs, cnt), dim3(256), 0, st, s->Z + 3 * L, sz, s->dy,
-                       srow, n, g);
+                       srow, n, s->part);
+    hipLaunchKernelGGL(sum_parts, dim3(nblk(H + 1, 64), cnt), dim3(256), 0, st, s->part, rs, H + 1,
+                       g, kW5);
     hipLaunchKernelGGL(out_bwd, dim3(nblk(L, 256), cnt), dim3(256), 0, st, s->Z + 3 * L, sz, p,
                        s->dy, srow, n, s->dZa, sdz);
     bf16* dz = s->dZa;
     bf16* dzn = s->dZb;
     for (int l = 2; l >= 0; --l) {  // hidden layer l+2: weights W_{l+2}, input H_{l+1}
-        const int splits = std::max(1, std::min(32, n / 512));
+        const int splits = std::max(1, std::min(kMaxSplits, n / 512));
         GemmArgs gw{};
         gw.A = dz; gw.sam = 1; gw.sak = H;               // A(o, m) = dZ[m][o]
         gw.B = s->Hb + l * L; gw.sbk = H; gw.sbn = 1;    // B(m, i) = H_{l+1}[m][i]
         gw.M = H; gw.N = H; gw.K = n;
-        gw.C = g + kW2 + l * (H * H + H);
+        gw.C = s->part;  // one H x H slot per (member, split)
         gw.k_split = ((n + splits - 1) / splits + 31) / 32 * 32;
         gw.splits = splits;
         gw.mA = sdz;
         gw.mB = sz;
-        gw.mC = kP;
         if (hipError_t e = launch_gemm(EPI_WGRAD, gw, st, cnt * splits)) return e;
+        hipLaunchKernelGGL(sum_parts, dim3(nblk(H * H, 64), cnt), dim3(256), 0, st, s->part,
+                           splits, H * H, g, kW2 + l * (H * H + H));
         hipLaunchKernelGGL(colsum_bf16, dim3(H / 64, rs, cnt), dim3(256), 0, st, dz, sdz, n,
-                           g + kB2 + l * (H * H + H));
+                           s->part);
+        hipLaunchKernelGGL(sum_parts, dim3(nblk(H, 64), cnt), dim3(256), 0, st, s->part, rs, H, g,
+                           kB2 + l * (H * H + H));
         GemmArgs gb{};
         gb.A = dz; gb.sam = H; gb.sak = 1;                     // A(m, o) = dZ[m][o]
         gb.B = wb + l * H * H; gb.sbk = H; gb.sbn = 1;         // B(o, i) = W[o][i]
@@ -698,7 +771,11 @@ hipError_t backward(kwgpu_surrogate* s, int m0, int cnt, int n, hipStream_t st) 
         if (hipError_t e = launch_gemm(EPI_BWD, gb, st, cnt)) return e;
         std::swap(dz, dzn);
     }
-    hipLaunchKernelGGL(l1_grad, dim3(H / 64, rs, cnt), dim3(256), 0, st, dz, sdz, s->x, sx, n, g);
+    const int rs1 = std::max(1, std::min(64, n / 128));
+    static_assert(64 <= kPartRows, "partial slots");
+    hipLaunchKernelGGL(l1_grad, dim3(rs1, cnt), dim3(256), 0, st, dz, sdz, s->x, sx, n, s->part);
+    hipLaunchKernelGGL(sum_parts, dim3(nblk(kPartWidth, 64), cnt), dim3(256), 0, st, s->part, rs1,
+                       kPartWidth, g, kW1);
     return hipGetLastError();
 }
 
@@ -754,6 +831,7 @@ int kwgpu_surrogate_create(kwgpu_ctx* ctx, int32_t members, uint64_t seed, kwgpu
     const size_t M = (size_t)members;
     if (hipMalloc(&s->pall, M * kP * sizeof(float)) ||
         hipMalloc(&s->wball, M * 3 * H * H * sizeof(bf16)) ||
+        hipMalloc(&s->part, M * kPartFloats * sizeof(float)) ||
         hipMalloc(&s->grad, M * kP * sizeof(float)) || hipMalloc(&s->m1, M * kP * sizeof(float)) ||
         hipMalloc(&s->m2, M * kP * sizeof(float)) || hipMalloc(&s->loss, M * sizeof(float))) {
         kwgpu_surrogate_destroy(s);
@@ -782,7 +860,7 @@ void kwgpu_surrogate_destroy(kwgpu_surrogate* s) {
     (void)hipDeviceSynchronize();
     free_scratch(s);
     for (void* q : {(void*)s->pall, (void*)s->wball, (void*)s->grad, (void*)s->m1, (void*)s->m2,
-                    (void*)s->loss, (void*)s->yall})
+                    (void*)s->loss, (void*)s->yall, (void*)s->part})
         if (q) (void)hipFree(q);
     delete s;
 }

@@ -99,14 +99,23 @@ def predict(ctx: Context, sur: Surrogate, d_opts, n: int):
     return d_pm, d_sp, d_ib
 
 
-def calibrate_spread(err, spread, bar=ERR_BAR, cap=SPREAD_MAX, margin=0.5):
+MIN_CALIBRATED = 1000
+
+
+def calibrate_spread(err, spread, bar=ERR_BAR, cap=SPREAD_MAX, margin=0.5,
+                     min_count=MIN_CALIBRATED):
     """Largest spread bound <= cap such that every held-out option at or under it has
-    |error| <= margin * bar; 0 when none qualifies."""
+    |error| <= margin * bar, and at least `min_count` held-out options are under it;
+    0 (serve nothing) otherwise. With k exchangeable held-out options under the bound,
+    a new option under it exceeds their largest error with probability <= 1/(k+1), so
+    the count is what makes the held-out maximum mean anything."""
     order = np.argsort(spread)
     s, e = spread[order], np.abs(err[order])
     ok = np.maximum.accumulate(e) <= margin * bar
     idx = np.nonzero(ok & (s <= cap))[0]
-    return float(s[idx[-1]]) if len(idx) else 0.0
+    if not len(idx) or idx[-1] + 1 < min_count:
+        return 0.0
+    return float(s[idx[-1]])
 
 
 def train_and_evaluate(ctx: Context, n_train=2_000_000, n_test=65536, members=4, epochs=30,

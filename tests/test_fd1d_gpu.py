@@ -452,6 +452,16 @@ def test_autotune_keeps_results(ctx):
     assert {b["bucket"] for b in rep["buckets"]} == {256, 512, 1024, 2048}
     for b in rep["buckets"]:
         assert all(c["ms"] > 0 for c in b["candidates"])
+        # a lane candidate left untimed carries a lower bound no faster than the best
+        # wave variant, so pruning never discards a winner
+        best_wave = min(c["ms"] for c in b["candidates"] if c["layout"] == "wave")
+        for c in b["candidates"]:
+            if c["layout"] == "lane":
+                assert isinstance(c["pruned"], bool)
+                if c["pruned"]:
+                    assert c["ms"] >= best_wave
+        assert b["candidates"][b["chosen"]]["layout"] == "wave" or \
+            not b["candidates"][b["chosen"]].get("pruned", False)
     c2.fd1d_price_device(d_o.data_ptr(), n, d_p.data_ptr(), d_s.data_ptr(), prm)
     c2.synchronize()
     after = d_p.cpu().numpy()

@@ -201,10 +201,12 @@ struct GemmArgs {
 };
 
 // C(M x N) = A(M x K) B(K x N), 64 x 64 tile per 256-thread workgroup, each wave a
-// 32 x 32 quarter as 2 x 2 MFMA 16x16x32 tiles; K in steps of 32 through LDS.
-template <int EPI>
+// 32 x 32 quarter as 2 x 2 MFMA 16x16x32 tiles; K in steps of BK through LDS.
+template <int EPI, int BK>
 __global__ __launch_bounds__(256) void gemm_bf16(GemmArgs g) {
-    constexpr int BM = 64, BN = 64, BK = 32, LD = BK + 8;
+    constexpr int BM = 64, BN = 64, LD = BK + 8;
+    static_assert(BK % 32 == 0 && (EPI != EPI_WGRAD || BK == 32),
+                  "k staged in 32-wide chunks; the transposing loader stages 32");
     __shared__ __attribute__((aligned(16))) bf16 As[BM * LD];
     __shared__ __attribute__((aligned(16))) bf16 Bs[BN * LD];
     const int t = threadIdx.x, w = t >> 6, l = t & 63;
@@ -255,45 +257,52 @@ __global__ __launch_bounds__(256) void gemm_bf16(GemmArgs g) {
                 Bs[(c8 + i) * LD + kk] = vb[i];
             }
         } else {
-        {
-            const int m = m0 + lr, k = k0 + lc;
-            bf16x8 v;
-            if (g.sak == 1 && m < g.M && k + 8 <= ke) {
-                v = *reinterpret_cast<const bf16x8*>(g.A + (long)m * g.sam + k);
-            } else {
+            // BK / 32 chunks of 16 bytes per thread and operand, all loads issued before
+            // the first LDS store
+            constexpr int KC = BK / 32;
+            bf16x8 va[KC], vb[KC];
 #pragma unroll
-                for (int i = 0; i < 8; ++i)
-                    v[i] = (m < g.M && k + i < ke) ? g.A[(long)m * g.sam + (long)(k + i) * g.sak]
-                                                   : (bf16)0.0f;
-            }
-            *reinterpret_cast<bf16x8*>(&As[lr * LD + lc]) = v;
-        }
-        {
-            const int n = n0 + lr, k = k0 + lc;
-            bf16x8 v;
-            if (g.sbk == 1 && n < g.N && k + 8 <= ke) {
-                v = *reinterpret_cast<const bf16x8*>(g.B + (long)n * g.sbn + k);
-            } else {
+            for (int c = 0; c < KC; ++c) {
+                const int m = m0 + lr, n = n0 + lr, k = k0 + 32 * c + lc;
+                if (g.sak == 1 && m < g.M && k + 8 <= ke) {
+                    va[c] = *reinterpret_cast<const bf16x8*>(g.A + (long)m * g.sam + k);
+                } else {
 #pragma unroll
-                for (int i = 0; i < 8; ++i)
-                    v[i] = (n < g.N && k + i < ke) ? g.B[(long)(k + i) * g.sbk + (long)n * g.sbn]
-                                                   : (bf16)0.0f;
+                    for (int i = 0; i < 8; ++i)
+                        va[c][i] = (m < g.M && k + i < ke)
+                                       ? g.A[(long)m * g.sam + (long)(k + i) * g.sak]
+                                       : (bf16)0.0f;
+                }
+                if (g.sbk == 1 && n < g.N && k + 8 <= ke) {
+                    vb[c] = *reinterpret_cast<const bf16x8*>(g.B + (long)n * g.sbn + k);
+                } else {
+#pragma unroll
+                    for (int i = 0; i < 8; ++i)
+                        vb[c][i] = (n < g.N && k + i < ke)
+                                       ? g.B[(long)(k + i) * g.sbk + (long)n * g.sbn]
+                                       : (bf16)0.0f;
+                }
             }
-            *reinterpret_cast<bf16x8*>(&Bs[lr * LD + lc]) = v;
-        }
+#pragma unroll
+            for (int c = 0; c < KC; ++c) {
+                *reinterpret_cast<bf16x8*>(&As[lr * LD + 32 * c + lc]) = va[c];
+                *reinterpret_cast<bf16x8*>(&Bs[lr * LD + 32 * c + lc]) = vb[c];
+            }
         }
         __syncthreads();
 #pragma unroll
-        for (int i = 0; i < 2; ++i) {
-            const bf16x8 a =
-                *reinterpret_cast<const bf16x8*>(&As[(wm + 16 * i + (l & 15)) * LD + 8 * (l >> 4)]);
+        for (int kk = 0; kk < BK; kk += 32)
 #pragma unroll
-            for (int j = 0; j < 2; ++j) {
-                const bf16x8 b = *reinterpret_cast<const bf16x8*>(
-                    &Bs[(wn + 16 * j + (l & 15)) * LD + 8 * (l >> 4)]);
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[i][j], 0, 0, 0);
+            for (int i = 0; i < 2; ++i) {
+                const bf16x8 a = *reinterpret_cast<const bf16x8*>(
+                    &As[(wm + 16 * i + (l & 15)) * LD + kk + 8 * (l >> 4)]);
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    const bf16x8 b = *reinterpret_cast<const bf16x8*>(
+                        &Bs[(wn + 16 * j + (l & 15)) * LD + kk + 8 * (l >> 4)]);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[i][j], 0, 0, 0);
+                }
             }
-        }
         __syncthreads();
     }
     // D layout of v_mfma_f32_16x16x32: lane l, element e -> row 4 (l / 16) + e, col l % 16
@@ -521,12 +530,21 @@ __global__ void adam_kernel(float* p, const float* g, float* m1, float* m2, int 
     p[i] -= lr * (a / c1) / (sqrtf(v / c2) + eps);
 }
 
-// BF16 copies of W2..W4: blockIdx.y = 3 * member + layer
-__global__ void to_bf16(const float* p, bf16* w) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= H * H) return;
+// BF16 copies of W2..W4 and of their transposes: blockIdx.y = 3 * member + layer
+// (one 16 x 16 tile per 256-thread block; the transpose goes through LDS so that both
+// copies are written in 32-byte row segments)
+__global__ __launch_bounds__(256) void to_bf16(const float* p, bf16* w, bf16* wT) {
+    static_assert(H % 16 == 0, "16 x 16 tiles");
+    __shared__ bf16 tile[16][17];
+    const int t = threadIdx.x, tr = blockIdx.x / (H / 16), tc = blockIdx.x % (H / 16);
     const int k = blockIdx.y / 3, l = blockIdx.y - 3 * k;
-    w[(long)k * 3 * H * H + (long)l * H * H + i] = (bf16)p[(long)k * kP + kW2 + l * (H * H + H) + i];
+    const long o = (long)k * 3 * H * H + (long)l * H * H;
+    const int r = tr * 16 + (t >> 4), c = tc * 16 + (t & 15);
+    const bf16 v = (bf16)p[(long)k * kP + kW2 + l * (H * H + H) + r * H + c];
+    w[o + r * H + c] = v;
+    tile[t >> 4][t & 15] = v;
+    __syncthreads();
+    wT[o + (long)(tc * 16 + (t >> 4)) * H + tr * 16 + (t & 15)] = tile[t & 15][t >> 4];  // WT[c][r]
 }
 
 __global__ void zero_f(float* p, size_t n) {
@@ -615,8 +633,10 @@ struct kwgpu_surrogate {
     int device = 0;
     float* pall = nullptr;             // FP32 master weights, member-major (kP each)
     bf16* wball = nullptr;             // BF16 W2 | W3 | W4, member-major (3 H^2 each)
+    bf16* wballT = nullptr;            // ... and their transposes (backward-data GEMMs)
     std::vector<float*> p;             // pall + m kP
     std::vector<bf16*> wb;             // wball + 3 m H^2
+    std::vector<bf16*> wbT;            // wballT + 3 m H^2
     float *grad = nullptr, *m1 = nullptr, *m2 = nullptr, *loss = nullptr;  // per member
     // per-member scratch slots of `cap` rows each (grown on demand)
     size_t cap = 0;
@@ -627,6 +647,8 @@ struct kwgpu_surrogate {
     size_t yall_cap = 0;
     float* part = nullptr;  // per-block partial sums of the column reductions (kPartRows
                             // blocks x kPartWidth per member)
+    void* serve = nullptr;  // kwgpu_surrogate_price's per-option buffers (grown on demand)
+    size_t serve_cap = 0;   // options they hold
 };
 
 namespace {
@@ -683,12 +705,15 @@ GemmArgs gemm_fwd(const bf16* Hin, const bf16* W, int n, const float* bias, floa
 
 hipError_t launch_gemm(int epi, const GemmArgs& g, hipStream_t st, int z) {
     dim3 grid(nblk(g.N, 64), nblk(g.M, 64), z);
+    // k in 32-wide stages: 64- and 128-wide stages (more loads in flight and MFMAs per
+    // barrier) raise the register count to 108 / 184 and measured slower (the layer
+    // GEMMs are bound by their FP32 pre-activation traffic, not by the operand loads)
     if (epi == EPI_FWD)
-        hipLaunchKernelGGL(gemm_bf16<EPI_FWD>, grid, dim3(256), 0, st, g);
+        hipLaunchKernelGGL((gemm_bf16<EPI_FWD, 32>), grid, dim3(256), 0, st, g);
     else if (epi == EPI_BWD)
-        hipLaunchKernelGGL(gemm_bf16<EPI_BWD>, grid, dim3(256), 0, st, g);
+        hipLaunchKernelGGL((gemm_bf16<EPI_BWD, 32>), grid, dim3(256), 0, st, g);
     else
-        hipLaunchKernelGGL(gemm_bf16<EPI_WGRAD>, grid, dim3(256), 0, st, g);
+        hipLaunchKernelGGL((gemm_bf16<EPI_WGRAD, 32>), grid, dim3(256), 0, st, g);
     return hipGetLastError();
 }
 
@@ -724,7 +749,7 @@ hipError_t forward(kwgpu_surrogate* s, int m0, int cnt, int n, bool shared_x, fl
 // (zeroed here).
 hipError_t backward(kwgpu_surrogate* s, int m0, int cnt, int n, hipStream_t st) {
     const float* p = s->p[m0];
-    const bf16* wb = s->wb[m0];
+    const bf16* wbT = s->wbT[m0];
     const long cap = (long)s->cap, sz = 4 * cap * H, sx = cap * F, sdz = cap * H, srow = cap;
     const size_t L = (size_t)n * H;
     float* g = s->grad;
@@ -760,7 +785,7 @@ hipError_t backward(kwgpu_surrogate* s, int m0, int cnt, int n, hipStream_t st) 
                            kB2 + l * (H * H + H));
         GemmArgs gb{};
         gb.A = dz; gb.sam = H; gb.sak = 1;                     // A(m, o) = dZ[m][o]
-        gb.B = wb + l * H * H; gb.sbk = H; gb.sbn = 1;         // B(o, i) = W[o][i]
+        gb.B = wbT + l * H * H; gb.sbk = 1; gb.sbn = H;       // B(o, i) = W[o][i] = WT[i][o]
         gb.M = n; gb.N = H; gb.K = H;
         gb.Z = s->Z + l * L;                                   // pre-activations of H_{l+1}
         gb.Hout = dzn;
@@ -781,8 +806,8 @@ hipError_t backward(kwgpu_surrogate* s, int m0, int cnt, int n, hipStream_t st) 
 
 // BF16 copies of members m0 .. m0+cnt-1
 hipError_t refresh_bf16(kwgpu_surrogate* s, int m0, int cnt, hipStream_t st) {
-    hipLaunchKernelGGL(to_bf16, dim3(nblk(H * H, 256), 3 * cnt), dim3(256), 0, st, s->p[m0],
-                       s->wb[m0]);
+    hipLaunchKernelGGL(to_bf16, dim3((H / 16) * (H / 16), 3 * cnt), dim3(256), 0, st, s->p[m0],
+                       s->wb[m0], s->wbT[m0]);
     return hipGetLastError();
 }
 
@@ -831,6 +856,7 @@ int kwgpu_surrogate_create(kwgpu_ctx* ctx, int32_t members, uint64_t seed, kwgpu
     const size_t M = (size_t)members;
     if (hipMalloc(&s->pall, M * kP * sizeof(float)) ||
         hipMalloc(&s->wball, M * 3 * H * H * sizeof(bf16)) ||
+        hipMalloc(&s->wballT, M * 3 * H * H * sizeof(bf16)) ||
         hipMalloc(&s->part, M * kPartFloats * sizeof(float)) ||
         hipMalloc(&s->grad, M * kP * sizeof(float)) || hipMalloc(&s->m1, M * kP * sizeof(float)) ||
         hipMalloc(&s->m2, M * kP * sizeof(float)) || hipMalloc(&s->loss, M * sizeof(float))) {
@@ -840,6 +866,7 @@ int kwgpu_surrogate_create(kwgpu_ctx* ctx, int32_t members, uint64_t seed, kwgpu
     for (int m = 0; m < members; ++m) {
         s->p.push_back(s->pall + (size_t)m * kP);
         s->wb.push_back(s->wball + (size_t)m * 3 * H * H);
+        s->wbT.push_back(s->wballT + (size_t)m * 3 * H * H);
         init_params(w, seed * 1000003ull + (uint64_t)m);
         if (hipMemcpy(s->p[m], w.data(), kP * sizeof(float), hipMemcpyHostToDevice)) {
             kwgpu_surrogate_destroy(s);
@@ -859,8 +886,9 @@ void kwgpu_surrogate_destroy(kwgpu_surrogate* s) {
     (void)hipSetDevice(s->device);
     (void)hipDeviceSynchronize();
     free_scratch(s);
-    for (void* q : {(void*)s->pall, (void*)s->wball, (void*)s->grad, (void*)s->m1, (void*)s->m2,
-                    (void*)s->loss, (void*)s->yall, (void*)s->part})
+    for (void* q : {(void*)s->pall, (void*)s->wball, (void*)s->wballT, (void*)s->grad,
+                    (void*)s->m1, (void*)s->m2,
+                    (void*)s->loss, (void*)s->yall, (void*)s->part, s->serve})
         if (q) (void)hipFree(q);
     delete s;
 }
@@ -1066,34 +1094,47 @@ int kwgpu_surrogate_price(kwgpu_ctx* ctx, kwgpu_surrogate* s, const kwgpu_option
     if (n > 0x7fffffffu)
         return kwgpu::ctx_fail(ctx, KWGPU_ERR_INVALID, "kwgpu_surrogate_price: too many options");
     hipStream_t st = kwgpu::ctx_stream(ctx);
-    double *prem = nullptr, *spr = nullptr, *sub_p = nullptr;
-    uint8_t* inb = nullptr;
-    uint32_t *idx = nullptr;
-    int* cnt = nullptr;
-    int32_t* sub_s = nullptr;
-    kwgpu_option* sub_o = nullptr;
-    int rc = KWGPU_OK;
-    auto cleanup = [&]() {
-        (void)hipStreamSynchronize(st);
-        for (void* q : {(void*)prem, (void*)spr, (void*)inb, (void*)idx, (void*)cnt,
-                        (void*)sub_p, (void*)sub_s, (void*)sub_o})
-            if (q) (void)hipFree(q);
+    // per-option buffers, carved from one allocation kept with the surrogate
+    auto carve = [n](char* base, size_t& at, size_t bytes_per) {
+        char* q = base + at;
+        at += (bytes_per * n + 255) / 256 * 256;
+        return q;
     };
-    if (hipMalloc(&prem, n * sizeof(double)) || hipMalloc(&spr, n * sizeof(double)) ||
-        hipMalloc(&inb, n) || hipMalloc(&idx, 2 * n * sizeof(uint32_t)) ||
-        hipMalloc(&cnt, 2 * sizeof(int)) || hipMalloc(&sub_p, n * sizeof(double)) ||
-        hipMalloc(&sub_s, n * sizeof(int32_t)) || hipMalloc(&sub_o, n * sizeof(kwgpu_option))) {
-        cleanup();
-        return kwgpu::ctx_fail(ctx, KWGPU_ERR_NOMEM, "kwgpu_surrogate_price: allocation");
+    size_t need = 0;
+    for (size_t b : {sizeof(double), sizeof(double), (size_t)1, 2 * sizeof(uint32_t), sizeof(double),
+                     sizeof(int32_t), sizeof(kwgpu_option)})
+        need += (b * n + 255) / 256 * 256;
+    need += 256;  // the two counters
+    if (s->serve_cap < n) {
+        (void)hipStreamSynchronize(st);
+        if (s->serve) (void)hipFree(s->serve);
+        s->serve = nullptr;
+        s->serve_cap = 0;
+        if (hipMalloc(&s->serve, need))
+            return kwgpu::ctx_fail(ctx, KWGPU_ERR_NOMEM, "kwgpu_surrogate_price: allocation");
+        s->serve_cap = n;
     }
+    char* base = (char*)s->serve;
+    size_t at = 0;
+    double* prem = (double*)carve(base, at, sizeof(double));
+    double* spr = (double*)carve(base, at, sizeof(double));
+    uint8_t* inb = (uint8_t*)carve(base, at, 1);
+    uint32_t* idx = (uint32_t*)carve(base, at, 2 * sizeof(uint32_t));
+    double* sub_p = (double*)carve(base, at, sizeof(double));
+    int32_t* sub_s = (int32_t*)carve(base, at, sizeof(int32_t));
+    kwgpu_option* sub_o = (kwgpu_option*)carve(base, at, sizeof(kwgpu_option));
+    int* cnt = (int*)(base + at);
+    int rc = KWGPU_OK;
+    auto cleanup = [&]() { (void)hipStreamSynchronize(st); };
     rc = kwgpu_surrogate_predict(ctx, s, d_opts, n, prem, spr, inb, st);
     if (rc) {
         cleanup();
         return rc;
     }
     hipLaunchKernelGGL(zero_f, dim3(1), dim3(64), 0, st, (float*)cnt, (size_t)2);
-    // exp(scale * sigma sqrt(T)) must stay below the FP32 range (~3.4e38 = e^88.7)
-    const double fp64_above = 80.0 / fd->scale;
+    // FP32-mixed fallback: exp(scale * sigma sqrt(T)) must stay below the FP32 range
+    // (~3.4e38 = e^88.7), so such options go to FP64; an FP64 fallback takes them all
+    const double fp64_above = fd->precision == KWGPU_FP64 ? 0.0 : 80.0 / fd->scale;
     hipLaunchKernelGGL(gate_kernel, dim3(nblk(n, 256)), dim3(256), 0, st, d_opts, n, prem, spr, inb,
                        spread_max, fp64_above, d_prices, d_route, cnt, idx, idx + n);
     int hc[2] = {0, 0};
@@ -1108,7 +1149,7 @@ int kwgpu_surrogate_price(kwgpu_ctx* ctx, kwgpu_surrogate* s, const kwgpu_option
         const uint32_t* ix = idx + (size_t)r * n;
         hipLaunchKernelGGL(gather_opts, dim3(nblk(m, 256)), dim3(256), 0, st, d_opts, ix, m, sub_o);
         kwgpu_fd1d_params p = *fd;
-        p.precision = r == 0 ? KWGPU_FP32_MIXED : KWGPU_FP64;
+        p.precision = r == 0 ? fd->precision : KWGPU_FP64;
         p.x_dims = nullptr;  // per-option sizes would need the same gather
         p.t_dims = nullptr;
         rc = kwgpu_fd1d_price_device(ctx, sub_o, (size_t)m, &p, sub_p, sub_s, st);

@@ -194,12 +194,16 @@ class Context:
                                     prices.ctypes.data, status.ctypes.data)
         return prices, status, rc, (self.last_error if rc else "")
 
-    def fd1d_bs_price(self, opts, x_dim=512, t_dim=512, **kw):
+    def fd1d_bs_price(self, opts, x_dim=512, t_dim=512, *, x_dims=None, t_dims=None, **kw):
+        """Fd1d_BlackScholes_Pricer::price: (prices, status, rc, error)."""
         a = as_options(opts)
         n = len(a)
         prices = np.full(n, np.nan)
         status = np.zeros(n, np.int32)
-        p = params(x_dim, t_dim, **kw)
+        xd = None if x_dims is None else np.ascontiguousarray(x_dims, np.int32)
+        td = None if t_dims is None else np.ascontiguousarray(t_dims, np.int32)
+        p = params(x_dim, t_dim, x_dims=None if xd is None else xd.ctypes.data,
+                   t_dims=None if td is None else td.ctypes.data, **kw)
         rc = lib().kwgpu_fd1d_bs_price(self.h, a.ctypes.data, n, ctypes.byref(p),
                                        prices.ctypes.data, status.ctypes.data)
         return prices, status, rc, (self.last_error if rc else "")

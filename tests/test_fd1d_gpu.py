@@ -80,6 +80,27 @@ def test_fd1d_bs_plain_512(ctx):
     assert np.max(np.abs(ctx.bs_price(e) - c["ref_bs_euro"])) <= 1e-12
 
 
+def test_fd1d_bs_uneven_grids(ctx):
+    """FD1D-BS with per-option grid sizes: both FD legs go through one solve of 2n
+    options, so each option's sizes must reach its European copy too; the oracle prices
+    each (x, t) group as its own FD1D-BS call (src/Pricer/kwFd1d_BlackScholes.cpp:15-43)."""
+    _say("fd1d_bs uneven")
+    o = W.plain(90, 23)
+    xd = np.array([256, 300, 512], np.int32)[np.arange(len(o)) % 3]
+    td = np.array([40, 64], np.int32)[np.arange(len(o)) % 2]
+    p, st, rc, err = ctx.fd1d_bs_price(o, x_dims=xd, t_dims=td)
+    assert rc == kwgpu.OK, err
+    assert np.all(st == 0)
+    want = np.full(len(o), np.nan)
+    for x in np.unique(xd):
+        for t in np.unique(td):
+            sel = (xd == x) & (td == t)
+            if sel.any():
+                want[sel], e2 = oracle.fd1d_bs_price(o[sel], int(x), int(t))
+                assert e2 == ""
+    assert np.max(np.abs(p - want)) <= EPS
+
+
 @pytest.mark.parametrize("precision", ["fp64", "uneven"])
 def test_greeks_match_oracle(ctx, precision):
     """Bump Greeks (test/portfolio.py:35-56) through the C-ABI against the oracle's: each

@@ -226,7 +226,15 @@ def test_serving_routes_and_fallback(ctx, tmp_path):
     p_all_fd = d_p.cpu().numpy()
     ref, _, rc, _ = ctx.fd1d_price(o, 256, 64, precision=kwgpu.FP64)
     assert rc == 0
-    assert np.max(np.abs(p_all_fd - ref)) <= 1e-4
+    # an FP64 fallback is the FP64 pricer itself
+    assert np.max(np.abs(p_all_fd - ref)) <= 1e-8
+    # an FP32-mixed fallback stays within its 1e-4 bar, on a smaller batch that reuses
+    # the buffers the first call grew
+    half = n // 2
+    fd32 = kwgpu.params(256, 64, precision=kwgpu.FP32_MIXED)
+    nfd = s.price(d_o.data_ptr(), half, -1.0, fd32, d_p.data_ptr(), d_r.data_ptr())
+    assert nfd == half
+    assert np.max(np.abs(d_p.cpu().numpy()[:half] - ref[:half])) <= 1e-4
     # everything in the box passes an infinite bound; out-of-box options still go to FD
     feats = kwgpu.surrogate_features(o)
     in_box = np.all(np.abs(feats) <= 1 + 1e-9, axis=1)

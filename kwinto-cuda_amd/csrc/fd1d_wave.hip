@@ -613,12 +613,24 @@ __device__ __forceinline__ void march_f64(double (&xv)[M], const double (&ap)[M]
                                           bool first_lane, bool last_lane, double* xchg,
                                           int wsteps, int steps, double c_th) {
     constexpr int NT = Geo<M, L, W>::NT;
+    // One option per workgroup (every lane marches every step): the loop is rotated so
+    // that each row's update is followed at once by that row's forward substitution for
+    // the next step. The substitution is a dependent chain of M FMAs; the row updates
+    // are independent of each other, so interleaved they fill the chain's latency.
+    constexpr bool ROT = Geo<M, L, W>::G == 1;
     const int wv = __builtin_amdgcn_readfirstlane(t.wv);  // wave of a W = 2 group
-    for (int step = 0; step < wsteps; ++step) {
-        double g[M];
+    double g[M];
+    if constexpr (ROT) {
         g[0] = xv[0];
 #pragma unroll
         for (int i = 1; i < M; ++i) g[i] = fma(-lo[i], g[i - 1], xv[i]);
+    }
+    for (int step = 0; step < wsteps; ++step) {
+        if constexpr (!ROT) {
+            g[0] = xv[0];
+#pragma unroll
+            for (int i = 1; i < M; ++i) g[i] = fma(-lo[i], g[i - 1], xv[i]);
+        }
         const double c = g[M - 1];
 #pragma unroll
         for (int i = M - 2; i >= 0; --i) g[i] = fma(-up[i], g[i + 1], g[i]);
@@ -645,9 +657,18 @@ __device__ __forceinline__ void march_f64(double (&xv)[M], const double (&ap)[M]
             gin = fwd_carry<L, W>(c, sm, t, first_lane, xchg);
             win = bwd_carry<L, W>(fma(R[0], gin, g[0]), sm, t, last_lane, xchg);
         }
-        // one option per workgroup: every step applies (no branch, so the LDS reads and
-        // fix-ups can be scheduled into the backward scan)
-        if (Geo<M, L, W>::G == 1 || step < steps) {
+        if constexpr (ROT) {
+            // no branch, so the LDS reads and fix-ups can also move into the backward scan
+#pragma unroll
+            for (int i = 0; i < M; ++i) {
+                const double2 qv = qpv[i * NT + t.gl];
+                double w = fma(R[i], gin, g[i]);
+                w = fma(qv.x, win, w);
+                const double v = HALF ? fma(ap[i], w, -xv[i]) : fma(ap[i], w, -c_th * xv[i]);
+                xv[i] = vmax(v, qv.y);
+                g[i] = i == 0 ? xv[0] : fma(-lo[i], g[i - 1], xv[i]);  // next step's
+            }
+        } else if (step < steps) {
 #pragma unroll
             for (int i = 0; i < M; ++i) {
                 const double2 qv = qpv[i * NT + t.gl];

@@ -750,7 +750,8 @@ int kwgpu_fd1d_bs_price(kwgpu_ctx* c, const kwgpu_option* opts, size_t n,
     if (int rc = check_params(c, p)) return rc;
     if (n == 0) return KWGPU_OK;
     if (!opts || !prices) return fail(c, KWGPU_ERR_INVALID, "kwgpu: null pointer");
-    if (n > 0x7fffffffu) return fail(c, KWGPU_ERR_INVALID, "kwgpu: too many options");
+    if (n > 0x3fffffffu)  // both legs are solved as one batch of 2n
+        return fail(c, KWGPU_ERR_INVALID, "kwgpu: too many options");
     if (int rc = check_dims(c, p->x_dims, p->t_dims, n)) return rc;
     if (hipError_t e = hipSetDevice(c->device); e != hipSuccess) return hip_fail(c, e, "hipSetDevice");
     const char* prefix = "Fd1d_BlackScholes_Pricer::price : ";

@@ -49,6 +49,16 @@
 namespace kwgpu {
 namespace {
 
+#ifndef KWGPU_PF1
+#define KWGPU_PF1 3  // one option per wave
+#endif
+#ifndef KWGPU_PF2
+#define KWGPU_PF2 0  // two-wave groups: read at the point of use (any buffer costs spills)
+#endif
+#ifndef KWGPU_PFG
+#define KWGPU_PFG 0  // several options per wave
+#endif
+
 constexpr int kRowShr = 0x110;   // + n, n in 1..15
 constexpr int kRowShl = 0x100;   // + n
 constexpr int kRowBcast15 = 0x142;
@@ -618,6 +628,14 @@ __device__ __forceinline__ void march_f64(double (&xv)[M], const double (&ap)[M]
     // the next step. The substitution is a dependent chain of M FMAs; the row updates
     // are independent of each other, so interleaved they fill the chain's latency.
     constexpr bool ROT = Geo<M, L, W>::G == 1;
+    // LDS reads of the (Q, payoff) pairs in flight ahead of the row updates. Measured on
+    // one box: the 1024-point one-wave kernel 9.81 ms at 3, 10.03 at 4, 10.06 at 1 (and
+    // 10.3 at 0); the 2048-point two-wave kernel 13.2 ms at 0, 15.6 at 1, 22.3 at 3,
+    // where the extra registers turn into spill reloads inside the step loop; the
+    // 256- and 512-point kernels (4 and 2 options per wave) 2.69 and 5.34 ms at 0, 2.88
+    // and 6.33 at 3.
+    constexpr int PF0 = W == 2 ? KWGPU_PF2 : ROT ? KWGPU_PF1 : KWGPU_PFG;
+    constexpr int PF = PF0 < M ? PF0 : M - 1;  // 0: each pair read where it is used
     const int wv = __builtin_amdgcn_readfirstlane(t.wv);  // wave of a W = 2 group
     double g[M];
     if constexpr (ROT) {
@@ -658,10 +676,22 @@ __device__ __forceinline__ void march_f64(double (&xv)[M], const double (&ap)[M]
             win = bwd_carry<L, W>(fma(R[0], gin, g[0]), sm, t, last_lane, xchg);
         }
         if constexpr (ROT) {
-            // no branch, so the LDS reads and fix-ups can also move into the backward scan
+            // no branch, so the LDS reads and fix-ups can also move into the backward
+            // scan; the (Q, payoff) pairs are read PF rows ahead of their use
+            double2 qbuf[PF > 0 ? PF : 1];
+#pragma unroll
+            for (int k = 0; k < PF; ++k) qbuf[k] = qpv[k * NT + t.gl];
 #pragma unroll
             for (int i = 0; i < M; ++i) {
-                const double2 qv = qpv[i * NT + t.gl];
+                double2 qv;
+                if constexpr (PF == 0) {
+                    qv = qpv[i * NT + t.gl];
+                } else {
+                    qv = qbuf[0];
+#pragma unroll
+                    for (int k = 0; k + 1 < PF; ++k) qbuf[k] = qbuf[k + 1];
+                    if (i + PF < M) qbuf[PF - 1] = qpv[(i + PF) * NT + t.gl];
+                }
                 double w = fma(R[i], gin, g[i]);
                 w = fma(qv.x, win, w);
                 const double v = HALF ? fma(ap[i], w, -xv[i]) : fma(ap[i], w, -c_th * xv[i]);
@@ -669,9 +699,20 @@ __device__ __forceinline__ void march_f64(double (&xv)[M], const double (&ap)[M]
                 g[i] = i == 0 ? xv[0] : fma(-lo[i], g[i - 1], xv[i]);  // next step's
             }
         } else if (step < steps) {
+            double2 qbuf[PF > 0 ? PF : 1];
+#pragma unroll
+            for (int k = 0; k < PF; ++k) qbuf[k] = qpv[k * NT + t.gl];
 #pragma unroll
             for (int i = 0; i < M; ++i) {
-                const double2 qv = qpv[i * NT + t.gl];
+                double2 qv;
+                if constexpr (PF == 0) {
+                    qv = qpv[i * NT + t.gl];
+                } else {
+                    qv = qbuf[0];
+#pragma unroll
+                    for (int k = 0; k + 1 < PF; ++k) qbuf[k] = qbuf[k + 1];
+                    if (i + PF < M) qbuf[PF - 1] = qpv[(i + PF) * NT + t.gl];
+                }
                 double w = fma(R[i], gin, g[i]);
                 w = fma(qv.x, win, w);
                 const double v = HALF ? fma(ap[i], w, -xv[i]) : fma(ap[i], w, -c_th * xv[i]);

@@ -411,7 +411,7 @@ def main():
             "frac": achieved / FP64_PEAK_TFLOPS,
             "traffic": traffic,
             "kernel": ("fd1d_lane_f64" if args.layout == "lane" else
-                       "fd1d_wave<M=16,L=64,W=1,PERSIST=false,MIXED=%s>"
+                       "fd1d_wave<M=16,L=64,W=1,PERSIST=false,MIXED=%s,GTH=false>"
                        % ("true" if mixed_chosen else "false")),
             "kernel_ms": avg_kernel_s * 1e3,
             "algorithmic_flops_per_launch": flops,

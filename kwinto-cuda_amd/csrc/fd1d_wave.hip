@@ -722,7 +722,9 @@ __device__ __forceinline__ void march_f64(double (&xv)[M], const double (&ap)[M]
     }
 }
 
-template <int M, int L, int W>
+// GTH: general theta. The theta = 1/2 kernel holds only its own march, so set-up values
+// that only the general march needs are not kept (and spilled) on its path.
+template <int M, int L, int W, bool GTH>
 __device__ __forceinline__ void solve_group_f64(const FdBatch& b, int group, double* qp,
                                                 double* xchg) {
     using Gm = Geo<M, L, W>;
@@ -771,14 +773,17 @@ __device__ __forceinline__ void solve_group_f64(const FdBatch& b, int group, dou
     if constexpr (W == 2) qpv[M * NT + t.gl] = make_double2(aw, bw);
     __syncthreads();
     const int steps = d.steps;
-    if (c_th == 1.0)
-        march_f64<M, L, W, true>(xv, ap, lo, up, R, qpv, sm, t, first_lane, last_lane, xchg,
-                                 wsteps, steps, c_th);
-    else
-        march_f64<M, L, W, false>(xv, ap, lo, up, R, qpv, sm, t, first_lane, last_lane, xchg,
-                                  wsteps, steps, c_th);
-    // the option's record is read again rather than kept in registers over the march
-    interpolate<M, L, W>(b, pde_of<M, L, W>(b, group, t), t, xv, qp);
+    // (the general form gives the same values at theta = 1/2: -c_th V is exactly -V)
+    march_f64<M, L, W, !GTH>(xv, ap, lo, up, R, qpv, sm, t, first_lane, last_lane, xchg, wsteps,
+                             steps, c_th);
+    // the option's record and lane indices are derived again rather than kept in
+    // registers over the march: the group and thread index go through an empty asm so
+    // that the compiler cannot reuse the set-up's values (which it would otherwise keep,
+    // and spill, across the loop)
+    int g2 = group, tid = threadIdx.x;
+    asm volatile("" : "+v"(g2), "+v"(tid));
+    const Lanes t2 = lanes_of<M, L, W>(tid);
+    interpolate<M, L, W>(b, pde_of<M, L, W>(b, g2, t2), t2, xv, qp);
 }
 
 // ---- FP32-mixed march ---------------------------------------------------------------
@@ -880,7 +885,7 @@ __device__ __forceinline__ void solve_group_mixed(const FdBatch& b, int group, d
 // number of waves per SIMD the register budget is cut for (2: <= 256 registers).
 // With PERSIST the grid is persistent: workgroups take group indices from `queue`
 // until all `ngroups` are done (every workgroup reaches the exit test).
-template <int M, int L, int W, int WPE, bool PERSIST, bool MIXED>
+template <int M, int L, int W, int WPE, bool PERSIST, bool MIXED, bool GTH>
 __global__ __attribute__((amdgpu_flat_work_group_size(64 * W, 64 * W),
                           amdgpu_waves_per_eu(WPE, WPE)))
 void fd1d_wave(FdBatch b, int* queue, int ngroups) {
@@ -891,7 +896,7 @@ void fd1d_wave(FdBatch b, int* queue, int ngroups) {
         if constexpr (MIXED)
             solve_group_mixed<M, L, W>(b, g, qp, xchg);
         else
-            solve_group_f64<M, L, W>(b, g, qp, xchg);
+            solve_group_f64<M, L, W, GTH>(b, g, qp, xchg);
     };
     if constexpr (!PERSIST) {
         run(blockIdx.x);
@@ -913,7 +918,7 @@ __global__ void zero_counters(int* c, int n) {
     if (i < n) c[i] = 0;
 }
 
-template <int M, int L, int W, bool MIXED>
+template <int M, int L, int W, bool MIXED, bool GTH>
 hipError_t launch_wave(const FdBatch& b, hipStream_t s, int* queue) {
     constexpr int G = Geo<M, L, W>::G;
     constexpr int WPE = 2;
@@ -925,13 +930,21 @@ hipError_t launch_wave(const FdBatch& b, hipStream_t s, int* queue) {
         if (hipGetDevice(&dev) == hipSuccess)
             (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
         grid = std::min(ngroups, cus * 4 * WPE / W);
-        hipLaunchKernelGGL((fd1d_wave<M, L, W, WPE, true, MIXED>), dim3(grid), dim3(64 * W), 0,
-                           s, b, queue, ngroups);
+        hipLaunchKernelGGL((fd1d_wave<M, L, W, WPE, true, MIXED, GTH>), dim3(grid), dim3(64 * W),
+                           0, s, b, queue, ngroups);
     } else {
-        hipLaunchKernelGGL((fd1d_wave<M, L, W, WPE, false, MIXED>), dim3(grid), dim3(64 * W), 0,
-                           s, b, queue, ngroups);
+        hipLaunchKernelGGL((fd1d_wave<M, L, W, WPE, false, MIXED, GTH>), dim3(grid), dim3(64 * W),
+                           0, s, b, queue, ngroups);
     }
     return hipGetLastError();
+}
+
+// The FP64 march has a theta = 1/2 kernel and a general one; the FP32-mixed march
+// handles any theta in one form.
+template <int M, int L, int W, bool MIXED>
+hipError_t launch_wave(const FdBatch& b, hipStream_t s, int* queue) {
+    if (!MIXED && b.theta != 0.5) return launch_wave<M, L, W, MIXED, !MIXED>(b, s, queue);
+    return launch_wave<M, L, W, MIXED, false>(b, s, queue);
 }
 
 template <bool MIXED>

@@ -271,6 +271,22 @@ def test_theta_general_matches_reference_at_half(ctx):
     assert e1024 < e64 / 6, (e64, e1024)
 
 
+@pytest.mark.parametrize("x_dim", [256, 1024, 2048])
+def test_general_theta_kernel_agrees_at_half(ctx, x_dim):
+    """The FP64 march is built as a theta = 1/2 kernel and a general-theta kernel (chosen
+    by the host on theta != 1/2). theta = 1/2 + 1e-12 runs the general kernel, whose
+    prices must match the theta = 1/2 kernel's to the FP64 bar: this covers the
+    multi-option (256), one-wave (1024) and two-wave (2048) general kernels."""
+    _say("general theta", x_dim)
+    o = W.plain(48, 61)
+    half, st, rc, err = ctx.fd1d_price(o, x_dim, 64)
+    assert rc == kwgpu.OK, err
+    gen, st2, rc, err = ctx.fd1d_price(o, x_dim, 64, theta=0.5 + 1e-12)
+    assert rc == kwgpu.OK, err
+    assert np.array_equal(st, st2)
+    assert np.max(np.abs(gen - half)) <= EPS
+
+
 def test_device_path_full_size_properties(ctx):
     """32768 options, 1024 x 1024, device-resident: head equals the golden vectors,
     American >= European >= 0, American >= K x the payoff interpolated between the nodes

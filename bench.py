@@ -52,13 +52,13 @@ def parse_args():
     ap.add_argument("--uneven", type=int, default=65536,
                     help="options of the uneven-grid batch measured beside the headline "
                          "(0 = skip)")
-    ap.add_argument("--surrogate", action="store_true",
+    ap.add_argument("--surrogate", type=int, default=1,
                     help="also train the premium ensemble on FD labels and time "
-                         "surrogate-first serving")
+                         "surrogate-first serving (0 = skip)")
     ap.add_argument("--sur-train", type=int, default=2_000_000,
                     help="labelled options (FD FP64 1024x1024); 5 %% of them are held out")
     ap.add_argument("--sur-serve", type=int, default=1 << 20)
-    ap.add_argument("--sur-epochs", type=int, default=30)
+    ap.add_argument("--sur-epochs", type=int, default=40)
     ap.add_argument("--next-rows", type=int, default=1,
                     help="also time FD1D-BS, a compressed chain book and Greeks (host entry "
                          "points, copies included)")
@@ -244,6 +244,7 @@ def bench_next_rows(ctx, opts, cpu_threads, log):
 
 
 def bench_surrogate(ctx, args, log):
+    import kwgpu
     from kwgpu import pipeline
     from kwgpu import workloads as W
     n_test = args.sur_train // 20  # 5 % of the labelled set held out
@@ -252,6 +253,16 @@ def bench_surrogate(ctx, args, log):
     opts = W.serving(args.sur_serve, 4)
     serve = pipeline.serve(ctx, sur, opts, train["spread_bound"], log=log)
     serve["meets_bar"] = serve["max_abs_err_vs_fp64_fd"] <= pipeline.ERR_BAR
+    # the same batch with per-option grids (x-points from {256, 512, 1024, 2048},
+    # t-points max(32, ceil(365 T))), judged by FP64 FD on those grids, with the FP64 and
+    # the FP32-mixed fallback
+    xd, td = W.serving_grids(opts, 5)
+    uneven = {}
+    for name, prec in (("fp64", kwgpu.FP64), ("fp32_mixed", kwgpu.FP32_MIXED)):
+        r = pipeline.serve(ctx, sur, opts, train["spread_bound"], log=log, x_dims=xd,
+                           t_dims=td, precision=prec)
+        r["meets_bar"] = r["max_abs_err_vs_fp64_fd"] <= pipeline.ERR_BAR
+        uneven[name + "_fallback"] = r
     # the trade-off at looser bars (not the metric: its bar is 1e-4)
     looser = {}
     for bar, v in train["looser_bars"].items():
@@ -259,7 +270,8 @@ def bench_surrogate(ctx, args, log):
         r["meets_bar"] = r["max_abs_err_vs_fp64_fd"] <= float(bar)
         looser[bar] = r
     sur.close()
-    return {"training": train, "serving": serve, "serving_at_looser_bars": looser}
+    return {"training": train, "serving": serve, "serving_per_option_grids": uneven,
+            "serving_at_looser_bars": looser}
 
 
 def main():

@@ -243,9 +243,14 @@ int kwgpu_surrogate_predict(kwgpu_ctx* ctx, kwgpu_surrogate* s, const kwgpu_opti
 /* Surrogate-first pricing: options in the box whose ensemble spread is <= spread_max
  * are priced as BS + premium; the rest go to the FD pricer with `fd` (fd->precision
  * FP64, or FP32_MIXED with options whose exp(scale sigma sqrt(T)) would overflow FP32
- * sent to FP64). d_route[i] is 0 for the surrogate, 1 for FD. *n_fd receives the number
- * routed to FD. Device pointers; synchronous (the FD subset is compacted on the
- * device). Per-option work buffers are kept with `s` and grown on demand. */
+ * sent to FP64). fd->x_dims / fd->t_dims, when set, are device arrays of n per-option
+ * grid sizes, and each routed option is solved on its own. d_route[i] is 0 for the
+ * surrogate, 1 for FD at fd->precision, 2 for FD in FP64 (with an FP64 fallback, every
+ * option sent to FD). Options sent to FD are solved in index order. A negative
+ * spread_max serves nothing and skips the ensemble.
+ * *n_fd receives the number routed to FD. Device pointers;
+ * synchronous (the FD subset is compacted on the device). Per-option work buffers are
+ * kept with `s` and grown on demand. */
 int kwgpu_surrogate_price(kwgpu_ctx* ctx, kwgpu_surrogate* s, const kwgpu_option* d_opts,
                           size_t n, double spread_max, const kwgpu_fd1d_params* fd,
                           double* d_prices, int32_t* d_route, size_t* n_fd);

@@ -172,13 +172,12 @@ __global__ void l1_fwd(const float* x, long sx, const float* p, int n, float* Z,
     const int k = blockIdx.y;
     x += k * sx;
     p += (long)k * kP;
-    Z += k * sz;
     Hb += k * sz;
     const int m = i / H, j = i - m * H;
     float z = p[kB1 + j];
 #pragma unroll
     for (int f = 0; f < F; ++f) z = fmaf(p[kW1 + j * F + f], x[m * F + f], z);
-    Z[i] = z;
+    if (Z) Z[k * sz + i] = z;  // prediction keeps no pre-activations
     Hb[i] = (bf16)silu(z);
 }
 
@@ -191,7 +190,7 @@ struct GemmArgs {
     long sbk, sbn;
     int M, N, K;
     const float* bias;  // EPI_FWD: bias[n]
-    float* Z;           // EPI_FWD: writes pre-activations; EPI_BWD: reads them
+    float* Z;           // EPI_FWD: writes pre-activations (unless null); EPI_BWD: reads them
     bf16* Hout;         // EPI_FWD: silu(Z); EPI_BWD: dZ = C * silu'(Z)
     float* C;           // EPI_WGRAD: C[z][m * N + n] = sum over this block's K range, one
                         // M x N slot per blockIdx.z (member, split), summed afterwards
@@ -221,7 +220,7 @@ __global__ __launch_bounds__(256) void gemm_bf16(GemmArgs g) {
     g.B += mem * g.mB;
     if constexpr (EPI == EPI_FWD) g.bias += mem * g.mBias;
     if constexpr (EPI != EPI_WGRAD) {
-        g.Z += mem * g.mZ;
+        if (g.Z) g.Z += mem * g.mZ;
         g.Hout += mem * g.mH;
     } else {
         g.C += (long)blockIdx.z * g.M * g.N;
@@ -319,7 +318,7 @@ __global__ __launch_bounds__(256) void gemm_bf16(GemmArgs g) {
                 const long o = (long)m * g.N + n;
                 if constexpr (EPI == EPI_FWD) {
                     const float z = v + g.bias[n];
-                    g.Z[o] = z;
+                    if (g.Z) g.Z[o] = z;
                     g.Hout[o] = (bf16)silu(z);
                 } else if constexpr (EPI == EPI_BWD) {
                     g.Hout[o] = (bf16)(v * dsilu(g.Z[o]));
@@ -329,8 +328,6 @@ __global__ __launch_bounds__(256) void gemm_bf16(GemmArgs g) {
             }
 }
 
-// output layer: y = b5 + W5 . silu(Z4) (FP32, one wave per sample); with targets,
-// dy = 2 (y - t) / n and the summed squared error.
 // output layer: y = silu(Z4) W5 + b5; with targets also dy = 2 (y - t) / n and the loss
 // sum. kOutSPW samples per wave; the loss is reduced in the block before one atomic.
 constexpr int kOutSPW = 8;
@@ -571,47 +568,119 @@ __global__ void combine_kernel(const kwgpu_option* opts, const float* y, int mem
 }
 
 // ---- serving ---------------------------------------------------------------------
-// Route 0: surrogate (in the box, ensemble spread <= spread_max): price = BS + premium.
+// Route 0: surrogate (in the box, ensemble spread <= spread_max): price = BS + premium;
+// premium == nullptr (no prediction made) routes every option to FD.
 // Route 1: FP32-mixed FD. Route 2: FP64 FD -- options whose far call boundary
 // exp(scale sigma sqrt(T)) would overflow the FP32 increments of the mixed march.
-__global__ void gate_kernel(const kwgpu_option* opts, size_t n, const double* premium,
-                            const double* spread, const uint8_t* in_box, double spread_max,
-                            double fp64_above, double* prices, int32_t* route, int* counts,
-                            uint32_t* idx1, uint32_t* idx2) {
+// Each block also counts its options of routes 1 and 2; the compaction below places
+// them in index order, so a batch routed the same way gives the same FD subset on
+// every call.
+constexpr int kGateBlock = 256;
+
+__global__ __launch_bounds__(kGateBlock) void gate_kernel(
+    const kwgpu_option* opts, size_t n, const double* premium, const double* spread,
+    const uint8_t* in_box, double spread_max, double fp64_above, double* prices,
+    int32_t* route, uint32_t* blk_counts) {
+    __shared__ uint32_t wc[2][kGateBlock / 64];
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    kwgpu_option o = opts[i];
-    int r;
-    double price = NAN;
-    if (in_box[i] && spread[i] <= spread_max && isfinite(premium[i])) {
-        kwgpu_option e = o;
-        e.e = 0;
-        price = bs_one(e) + premium[i];
-        r = isfinite(price) ? 0 : 1;
-    } else {
-        r = 1;
+    int r = -1;
+    if (i < n) {
+        const kwgpu_option o = opts[i];
+        double price = NAN;
+        if (premium && in_box[i] && spread[i] <= spread_max && isfinite(premium[i])) {
+            kwgpu_option e = o;
+            e.e = 0;
+            price = bs_one(e) + premium[i];
+            r = isfinite(price) ? 0 : 1;
+        } else {
+            r = 1;
+        }
+        if (r != 0) r = (o.z * sqrt(o.t) > fp64_above) ? 2 : 1;
+        prices[i] = price;
+        route[i] = r;
     }
-    if (r != 0) r = (o.z * sqrt(o.t) > fp64_above) ? 2 : 1;
-    // compaction with one atomic per wave and route
-    const int lane = threadIdx.x & 63;
-    const uint64_t below = (1ull << lane) - 1;
-#pragma unroll
-    for (int k = 1; k <= 2; ++k) {
-        const uint64_t votes = __ballot(r == k);
-        if (votes == 0) continue;
-        const int leader = __ffsll((long long)votes) - 1;
-        int base = 0;
-        if (lane == leader) base = atomicAdd(&counts[k - 1], __popcll(votes));
-        base = __shfl(base, leader);
-        if (r == k) (k == 1 ? idx1 : idx2)[base + __popcll(votes & below)] = (uint32_t)i;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint64_t v1 = __ballot(r == 1), v2 = __ballot(r == 2);
+    if (lane == 0) {
+        wc[0][w] = (uint32_t)__popcll(v1);
+        wc[1][w] = (uint32_t)__popcll(v2);
     }
-    prices[i] = price;
-    route[i] = r;
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        uint32_t c = 0;
+        for (int q = 0; q < kGateBlock / 64; ++q) c += wc[threadIdx.x][q];
+        blk_counts[2 * blockIdx.x + threadIdx.x] = c;
+    }
+}
+
+// Exclusive prefix sums of the per-block counts (in place, one workgroup) and the two
+// totals.
+__global__ __launch_bounds__(1024) void scan_block_counts(uint32_t* blk_counts, unsigned nb,
+                                                          int* totals) {
+    __shared__ uint32_t s1[1024], s2[1024];
+    const unsigned t = threadIdx.x;
+    const unsigned per = (nb + 1023) / 1024;
+    const unsigned b0 = min(nb, t * per), b1 = min(nb, b0 + per);
+    uint32_t a1 = 0, a2 = 0;
+    for (unsigned b = b0; b < b1; ++b) {
+        a1 += blk_counts[2 * b];
+        a2 += blk_counts[2 * b + 1];
+    }
+    s1[t] = a1;
+    s2[t] = a2;
+    __syncthreads();
+    for (unsigned d = 1; d < 1024; d <<= 1) {
+        const uint32_t x1 = t >= d ? s1[t - d] : 0, x2 = t >= d ? s2[t - d] : 0;
+        __syncthreads();
+        s1[t] += x1;
+        s2[t] += x2;
+        __syncthreads();
+    }
+    uint32_t o1 = s1[t] - a1, o2 = s2[t] - a2;
+    for (unsigned b = b0; b < b1; ++b) {
+        const uint32_t c1 = blk_counts[2 * b], c2 = blk_counts[2 * b + 1];
+        blk_counts[2 * b] = o1;
+        blk_counts[2 * b + 1] = o2;
+        o1 += c1;
+        o2 += c2;
+    }
+    if (t == 1023) {
+        totals[0] = (int)s1[1023];
+        totals[1] = (int)s2[1023];
+    }
+}
+
+// Index lists of routes 1 and 2 in index order, from the scanned block offsets.
+__global__ __launch_bounds__(kGateBlock) void compact_routes(const int32_t* route, size_t n,
+                                                             const uint32_t* blk_off,
+                                                             uint32_t* idx1, uint32_t* idx2) {
+    __shared__ uint32_t wc[2][kGateBlock / 64];
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = i < n ? route[i] : 0;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint64_t v1 = __ballot(r == 1), v2 = __ballot(r == 2);
+    if (lane == 0) {
+        wc[0][w] = (uint32_t)__popcll(v1);
+        wc[1][w] = (uint32_t)__popcll(v2);
+    }
+    __syncthreads();
+    if (r == 1 || r == 2) {
+        const int k = r - 1;
+        uint32_t base = blk_off[2 * blockIdx.x + k];
+        for (int q = 0; q < w; ++q) base += wc[k][q];
+        const uint64_t below = (1ull << lane) - 1;
+        (k == 0 ? idx1 : idx2)[base + __popcll((k == 0 ? v1 : v2) & below)] = (uint32_t)i;
+    }
 }
 
 __global__ void gather_opts(const kwgpu_option* opts, const uint32_t* idx, int n, kwgpu_option* out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) out[i] = opts[idx[i]];
+}
+
+__global__ void gather_dims(const int32_t* dims, const uint32_t* idx, int n, int32_t* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = dims[idx[i]];
 }
 
 __global__ void scatter_prices(const double* sub, const int32_t* sub_status, const uint32_t* idx,
@@ -728,10 +797,14 @@ hipError_t forward(kwgpu_surrogate* s, int m0, int cnt, int n, bool shared_x, fl
     float* Z = s->Z;
     bf16* Hb = s->Hb;
     const size_t L = (size_t)n * H;
-    hipLaunchKernelGGL(l1_fwd, dim3(nblk(L, 256), cnt), dim3(256), 0, st, s->x, sx, p, n, Z, Hb, sz);
+    // without targets (prediction) no backward follows: only the last layer's FP32
+    // pre-activations, which the output layer reads, are stored
+    const bool keep = with_targets;
+    hipLaunchKernelGGL(l1_fwd, dim3(nblk(L, 256), cnt), dim3(256), 0, st, s->x, sx, p, n,
+                       keep ? Z : nullptr, Hb, sz);
     for (int l = 0; l < 3; ++l) {
         GemmArgs g = gemm_fwd(Hb + l * L, wb + l * H * H, n, p + kB2 + l * (H * H + H),
-                              Z + (l + 1) * L, Hb + (l + 1) * L);
+                              keep || l == 2 ? Z + (l + 1) * L : nullptr, Hb + (l + 1) * L);
         g.mA = sz;
         g.mB = 3 * H * H;
         g.mBias = kP;
@@ -1102,7 +1175,8 @@ int kwgpu_surrogate_price(kwgpu_ctx* ctx, kwgpu_surrogate* s, const kwgpu_option
     };
     size_t need = 0;
     for (size_t b : {sizeof(double), sizeof(double), (size_t)1, 2 * sizeof(uint32_t), sizeof(double),
-                     sizeof(int32_t), sizeof(kwgpu_option)})
+                     sizeof(int32_t), sizeof(kwgpu_option), sizeof(int32_t), sizeof(int32_t),
+                     sizeof(uint32_t)})
         need += (b * n + 255) / 256 * 256;
     need += 256;  // the two counters
     if (s->serve_cap < n) {
@@ -1120,23 +1194,34 @@ int kwgpu_surrogate_price(kwgpu_ctx* ctx, kwgpu_surrogate* s, const kwgpu_option
     double* spr = (double*)carve(base, at, sizeof(double));
     uint8_t* inb = (uint8_t*)carve(base, at, 1);
     uint32_t* idx = (uint32_t*)carve(base, at, 2 * sizeof(uint32_t));
+    uint32_t* blk = (uint32_t*)carve(base, at, sizeof(uint32_t));  // 2 per 256 options
     double* sub_p = (double*)carve(base, at, sizeof(double));
     int32_t* sub_s = (int32_t*)carve(base, at, sizeof(int32_t));
     kwgpu_option* sub_o = (kwgpu_option*)carve(base, at, sizeof(kwgpu_option));
+    int32_t* sub_x = (int32_t*)carve(base, at, sizeof(int32_t));
+    int32_t* sub_t = (int32_t*)carve(base, at, sizeof(int32_t));
     int* cnt = (int*)(base + at);
     int rc = KWGPU_OK;
     auto cleanup = [&]() { (void)hipStreamSynchronize(st); };
-    rc = kwgpu_surrogate_predict(ctx, s, d_opts, n, prem, spr, inb, st);
-    if (rc) {
-        cleanup();
-        return rc;
+    // a negative (or NaN) bound passes no option: the ensemble is not evaluated
+    const bool serve_none = !(spread_max >= 0.0);
+    if (!serve_none) {
+        rc = kwgpu_surrogate_predict(ctx, s, d_opts, n, prem, spr, inb, st);
+        if (rc) {
+            cleanup();
+            return rc;
+        }
     }
-    hipLaunchKernelGGL(zero_f, dim3(1), dim3(64), 0, st, (float*)cnt, (size_t)2);
     // FP32-mixed fallback: exp(scale * sigma sqrt(T)) must stay below the FP32 range
     // (~3.4e38 = e^88.7), so such options go to FP64; an FP64 fallback takes them all
     const double fp64_above = fd->precision == KWGPU_FP64 ? 0.0 : 80.0 / fd->scale;
-    hipLaunchKernelGGL(gate_kernel, dim3(nblk(n, 256)), dim3(256), 0, st, d_opts, n, prem, spr, inb,
-                       spread_max, fp64_above, d_prices, d_route, cnt, idx, idx + n);
+    const unsigned nb = nblk(n, kGateBlock);
+    hipLaunchKernelGGL(gate_kernel, dim3(nb), dim3(kGateBlock), 0, st, d_opts, n,
+                       serve_none ? nullptr : prem, spr, inb, spread_max, fp64_above, d_prices,
+                       d_route, blk);
+    hipLaunchKernelGGL(scan_block_counts, dim3(1), dim3(1024), 0, st, blk, nb, cnt);
+    hipLaunchKernelGGL(compact_routes, dim3(nb), dim3(kGateBlock), 0, st, d_route, n, blk, idx,
+                       idx + n);
     int hc[2] = {0, 0};
     if (hipMemcpyAsync(hc, cnt, sizeof(hc), hipMemcpyDeviceToHost, st) ||
         hipStreamSynchronize(st)) {
@@ -1150,8 +1235,17 @@ int kwgpu_surrogate_price(kwgpu_ctx* ctx, kwgpu_surrogate* s, const kwgpu_option
         hipLaunchKernelGGL(gather_opts, dim3(nblk(m, 256)), dim3(256), 0, st, d_opts, ix, m, sub_o);
         kwgpu_fd1d_params p = *fd;
         p.precision = r == 0 ? fd->precision : KWGPU_FP64;
-        p.x_dims = nullptr;  // per-option sizes would need the same gather
-        p.t_dims = nullptr;
+        // per-option grid sizes follow their options into the compacted subset
+        if (fd->x_dims) {
+            hipLaunchKernelGGL(gather_dims, dim3(nblk(m, 256)), dim3(256), 0, st, fd->x_dims, ix, m,
+                               sub_x);
+            p.x_dims = sub_x;
+        }
+        if (fd->t_dims) {
+            hipLaunchKernelGGL(gather_dims, dim3(nblk(m, 256)), dim3(256), 0, st, fd->t_dims, ix, m,
+                               sub_t);
+            p.t_dims = sub_t;
+        }
         rc = kwgpu_fd1d_price_device(ctx, sub_o, (size_t)m, &p, sub_p, sub_s, st);
         if (rc) break;
         hipLaunchKernelGGL(scatter_prices, dim3(nblk(m, 256)), dim3(256), 0, st, sub_p, sub_s, ix, m,

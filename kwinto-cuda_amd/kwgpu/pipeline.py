@@ -6,9 +6,11 @@ and timing events only.
 
 Workloads (kwgpu.workloads): the training set is `surrogate_train` (K U[40,200],
 T U[1/52,5], the plain-method sigma/r/q/w draws, seed 3); the serving batch is
-`serving` (the same box with ~10% of options moved outside it, seed 4). Labels and the
-FD fallback use one grid, 1024 x-points x 1024 t-points, so served prices and the FP64
-FD prices they are judged against describe the same discretisation.
+`serving` (the same box with ~10% of options moved outside it, seed 4). Labels use one
+grid, 1024 x-points x 1024 t-points. Serving uses that grid for the FD fallback too, so
+served prices and the FP64 FD prices they are judged against describe the same
+discretisation, or per-option grids (`serve(..., x_dims, t_dims)`), against which FP64
+FD on those same grids is the judge.
 """
 from __future__ import annotations
 
@@ -106,7 +108,8 @@ def calibrate_spread(err, spread, bar=ERR_BAR, cap=SPREAD_MAX, margin=0.5,
                      min_count=MIN_CALIBRATED):
     """Largest spread bound <= cap such that every held-out option at or under it has
     |error| <= margin * bar, and at least `min_count` held-out options are under it;
-    0 (serve nothing) otherwise. With k exchangeable held-out options under the bound,
+    -1 (a bound no spread meets: serve nothing, and the pricer skips the ensemble)
+    otherwise. With k exchangeable held-out options under the bound,
     a new option under it exceeds their largest error with probability <= 1/(k+1), so
     the count is what makes the held-out maximum mean anything."""
     order = np.argsort(spread)
@@ -114,7 +117,7 @@ def calibrate_spread(err, spread, bar=ERR_BAR, cap=SPREAD_MAX, margin=0.5,
     ok = np.maximum.accumulate(e) <= margin * bar
     idx = np.nonzero(ok & (s <= cap))[0]
     if not len(idx) or idx[-1] + 1 < min_count:
-        return 0.0
+        return -1.0
     return float(s[idx[-1]])
 
 
@@ -166,16 +169,35 @@ def train_and_evaluate(ctx: Context, n_train=2_000_000, n_test=65536, members=4,
     return sur, out
 
 
-def serve(ctx: Context, sur: Surrogate, opts, spread_bound: float, repeats=3, log=print):
+def serve(ctx: Context, sur: Surrogate, opts, spread_bound: float, repeats=3, log=print,
+          x_dims=None, t_dims=None, precision=FP64):
     """Surrogate-first pricing of `opts` against plain FP64 FD of the same batch:
-    options/s of both, max/RMS error of the served prices vs FP64 FD."""
+    options/s of both, max/RMS error of the served prices vs FP64 FD. `x_dims`/`t_dims`
+    (host int32 arrays, optional) give every option its own grid, for the FD fallback and
+    for the FP64 FD judge alike. `precision` is the fallback's; FP64 is the default since
+    on MI355X the FP64 form is both exact and the faster one."""
     torch = _torch()
     n = len(opts)
     d_o = to_device(opts)
     d_p = torch.empty(n, dtype=torch.float64, device="cuda")
     d_r = torch.empty(n, dtype=torch.int32, device="cuda")
-    # FD fallback in FP64: on MI355X the FP64 form is both exact and the faster one
-    fd = params(X_DIM, T_DIM, precision=FP64)
+    dims = {}
+    if x_dims is not None or t_dims is not None:
+        xd = np.full(n, X_DIM, np.int32) if x_dims is None else np.asarray(x_dims, np.int32)
+        td = np.full(n, T_DIM, np.int32) if t_dims is None else np.asarray(t_dims, np.int32)
+        assert xd.shape == (n,) and td.shape == (n,)
+        d_x = torch.from_numpy(xd.copy()).cuda()
+        d_t = torch.from_numpy(td.copy()).cuda()
+        dims = {"x_dims": d_x.data_ptr(), "t_dims": d_t.data_ptr()}
+    fd = params(X_DIM, T_DIM, precision=precision, **dims)
+    fd64 = params(X_DIM, T_DIM, precision=FP64, **dims)
+    d_ref = torch.empty(n, dtype=torch.float64, device="cuda")
+    d_st = torch.empty(n, dtype=torch.int32, device="cuda")
+
+    def fd_ref():
+        ctx.fd1d_price_device(d_o.data_ptr(), n, d_ref.data_ptr(), d_st.data_ptr(), fd64)
+        ctx.synchronize()
+
     sur.price(d_o.data_ptr(), n, spread_bound, fd, d_p.data_ptr(), d_r.data_ptr())  # warm-up
     torch.cuda.synchronize()
     ts = []
@@ -188,15 +210,17 @@ def serve(ctx: Context, sur: Surrogate, opts, spread_bound: float, repeats=3, lo
     t_sur = min(ts)
     served = d_p.cpu().numpy()
     route = d_r.cpu().numpy()
-    fd_prices(ctx, d_o, n, FP64)  # warm-up, as for the surrogate-first path
+    fd_ref()  # warm-up, as for the surrogate-first path
     torch.cuda.synchronize()
     ts = []
     for _ in range(repeats):
         t0 = time.perf_counter()
-        d_ref = fd_prices(ctx, d_o, n, FP64)
+        fd_ref()
         torch.cuda.synchronize()
         ts.append(time.perf_counter() - t0)
     t_fd = min(ts)
+    if int((d_st != 0).sum().item()):
+        raise RuntimeError("FP64 FD judge: failed options")
     ref = d_ref.cpu().numpy()
     d = np.abs(served - ref)
     out = {

@@ -118,6 +118,15 @@ def serving(n: int = 1 << 20, seed: int = 4, outside_every: int = 10):
     return opts
 
 
+def serving_grids(opts, seed: int = 5):
+    """Per-option grid sizes for a serving batch, by the uneven batch's rule: x-points
+    from X_BUCKETS by the top two bits of a SplitMix64 stream of its own (so the options
+    themselves are unchanged), t-points = max(32, ceil(365 T))."""
+    raw = splitmix64(seed, len(opts))
+    x_dims = X_BUCKETS[(raw >> np.uint64(62)).astype(np.int64)].astype(np.int32)
+    return x_dims, t_points_for(opts["t"])
+
+
 def t_points_for(t: np.ndarray) -> np.ndarray:
     return np.maximum(32, np.ceil(365.0 * np.asarray(t))).astype(np.int32)
 

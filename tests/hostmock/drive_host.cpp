@@ -1,0 +1,242 @@
+// Drives every host-side path of the C-ABI in the order the GPU tests use them, on one
+// context, against the stand-in runtime of hip_mock.cpp (no kernel runs): uneven-batch
+// planning, replays and re-plans, PDE grouping, FD1D-BS and Greeks batches, the lane
+// workspace, the autotuner and its file, error returns, and the surrogate's training,
+// prediction and gated serving. Exits non-zero on an unexpected return code or on
+// allocations, streams, events or graphs left alive at the end. Test infrastructure only.
+#include <cstdio>
+#include <cstdlib>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "kwgpu.h"
+
+extern "C" int mock_live();
+
+namespace {
+
+int failures = 0;
+
+void expect(int rc, int want, kwgpu_ctx* c, const char* what) {
+    if (rc != want) {
+        std::fprintf(stderr, "FAIL %s: rc %d (want %d) %s\n", what, rc, want,
+                     c ? kwgpu_last_error(c) : "");
+        ++failures;
+    }
+}
+
+std::vector<kwgpu_option> options(size_t n, unsigned seed) {
+    std::mt19937_64 g(seed);
+    std::uniform_real_distribution<double> u(0.0, 1.0);
+    std::vector<kwgpu_option> o(n);
+    for (auto& x : o) {
+        x.t = 1.0 / 12 + u(g) * (3.0 - 1.0 / 12);
+        x.k = 50 + 100 * u(g);
+        x.z = 0.1 + 0.5 * u(g);
+        x.r = 0.1 * u(g);
+        x.q = 0.08 * u(g);
+        x.s = 100;
+        x.e = 1;
+        x.w = u(g) < 0.5 ? -1 : 1;
+    }
+    return o;
+}
+
+struct Sizes {
+    std::vector<int32_t> x, t;
+};
+
+Sizes sizes(size_t n, std::initializer_list<int> xs, int t0, int tspan, unsigned salt) {
+    Sizes s;
+    std::vector<int> xv(xs);
+    for (size_t i = 0; i < n; ++i) {
+        s.x.push_back(xv[(i * 7 + salt) % xv.size()]);
+        s.t.push_back(t0 + (int)((i * 13 + salt) % (size_t)tspan));
+    }
+    return s;
+}
+
+int host_price(kwgpu_ctx* c, const std::vector<kwgpu_option>& o, kwgpu_fd1d_params p) {
+    std::vector<double> pr(o.size());
+    std::vector<int32_t> st(o.size());
+    return kwgpu_fd1d_price(c, o.data(), o.size(), &p, pr.data(), st.data());
+}
+
+int device_price(kwgpu_ctx* c, const std::vector<kwgpu_option>& o, const kwgpu_fd1d_params& p) {
+    std::vector<double> pr(o.size());
+    std::vector<int32_t> st(o.size());
+    return kwgpu_fd1d_price_device(c, o.data(), o.size(), &p, pr.data(), st.data(), nullptr);
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    const std::string dir = argc > 1 ? argv[1] : ".";
+    kwgpu_ctx* c = nullptr;
+    expect(kwgpu_create(&c, 0), KWGPU_OK, c, "create");
+    kwgpu_fd1d_params p;
+
+    {  // FD1D-BS with per-option grids: one solve of 2n PDEs
+        auto o = options(90, 23);
+        Sizes s = sizes(90, {256, 300, 512}, 40, 25, 0);
+        kwgpu_fd1d_params_default(&p);
+        p.x_dims = s.x.data();
+        p.t_dims = s.t.data();
+        std::vector<double> pr(90);
+        std::vector<int32_t> st(90);
+        expect(kwgpu_fd1d_bs_price(c, o.data(), 90, &p, pr.data(), st.data()), KWGPU_OK, c,
+               "fd1d_bs uneven");
+    }
+    {  // Greeks: 9 bumped copies per option, uniform and per-option grids
+        auto o = options(96, 17);
+        std::vector<kwgpu_greeks> g(96);
+        std::vector<int32_t> st(96);
+        kwgpu_fd1d_params_default(&p);
+        expect(kwgpu_fd1d_greeks(c, o.data(), 96, &p, 0.01, g.data(), st.data()), KWGPU_OK, c,
+               "greeks");
+        Sizes s = sizes(96, {256, 512}, 128, 73, 1);
+        p.x_dims = s.x.data();
+        p.t_dims = s.t.data();
+        expect(kwgpu_fd1d_greeks(c, o.data(), 96, &p, 0.01, g.data(), st.data()), KWGPU_OK, c,
+               "greeks uneven");
+    }
+    {  // uneven host batches, grids above 2048 points in the lane layout (FP64 only)
+        auto o = options(40, 2);
+        Sizes s = sizes(40, {256, 512, 1024, 2048, 3000}, 32, 300, 2);
+        kwgpu_fd1d_params_default(&p);
+        p.x_dims = s.x.data();
+        p.t_dims = s.t.data();
+        expect(host_price(c, o, p), KWGPU_OK, c, "uneven with lane bucket");
+        p.precision = KWGPU_FP32_MIXED;
+        expect(host_price(c, o, p), KWGPU_ERR_UNSUPPORTED, c, "mixed above 2048 points");
+        Sizes s2 = sizes(40, {256, 512, 1024, 2048}, 32, 300, 3);
+        p.x_dims = s2.x.data();
+        p.t_dims = s2.t.data();
+        expect(host_price(c, o, p), KWGPU_OK, c, "mixed uneven");
+    }
+    for (int x : {3, 17, 64, 100, 129, 256, 300, 511, 1000, 1024, 2048, 4096})  // uniform grids
+        for (int lay : {KWGPU_LAYOUT_AUTO, KWGPU_LAYOUT_WAVE_REG, KWGPU_LAYOUT_LANE_SOA}) {
+            kwgpu_fd1d_params_default(&p);
+            p.x_dim = x;
+            p.t_dim = 20;
+            p.layout = lay;
+            const int want = lay == KWGPU_LAYOUT_WAVE_REG && x > 2048 ? KWGPU_ERR_UNSUPPORTED : KWGPU_OK;
+            expect(host_price(c, options(37, 1000 + x), p), want, c, "uniform sizes");
+        }
+    {  // device entry point: a replayed batch, then more batches than any cache would hold
+        auto o = options(3000, 77);
+        Sizes s = sizes(3000, {256, 512, 1024, 2048}, 32, 33, 4);
+        kwgpu_fd1d_params_default(&p);
+        p.x_dims = s.x.data();
+        p.t_dims = s.t.data();
+        for (int k = 0; k < 3; ++k) expect(device_price(c, o, p), KWGPU_OK, c, "device uneven");
+        std::vector<std::vector<kwgpu_option>> os;
+        std::vector<Sizes> ss;
+        for (int b = 0; b < 6; ++b) {
+            os.push_back(options(700 + 50 * b, 80 + b));
+            ss.push_back(sizes(700 + 50 * b, {256, 512, 1024, 2048}, 20, 21, 5 + b));
+        }
+        for (int b : {0, 1, 0, 1, 2, 3, 0, 1, 2, 3, 4, 5, 0, 5, 4, 1}) {
+            p.x_dims = ss[b].x.data();
+            p.t_dims = ss[b].t.data();
+            expect(device_price(c, os[b], p), KWGPU_OK, c, "device batches in turn");
+        }
+        // autotuner on the first batch, its file, and pricing with the loaded choices
+        p.x_dims = s.x.data();
+        p.t_dims = s.t.data();
+        std::vector<double> pr(o.size());
+        std::vector<int32_t> st(o.size());
+        std::vector<char> report(1 << 16);
+        expect(kwgpu_fd1d_autotune(c, o.data(), o.size(), &p, pr.data(), st.data(), report.data(),
+                                   report.size()),
+               KWGPU_OK, c, "autotune");
+        const std::string path = dir + "/tune.txt";
+        expect(kwgpu_tune_save(c, path.c_str()), KWGPU_OK, c, "tune save");
+        expect(device_price(c, o, p), KWGPU_OK, c, "tuned price");
+        kwgpu_ctx* c2 = nullptr;
+        expect(kwgpu_create(&c2, 0), KWGPU_OK, c2, "second context");
+        expect(kwgpu_tune_load(c2, path.c_str()), KWGPU_OK, c2, "tune load");
+        expect(device_price(c2, o, p), KWGPU_OK, c2, "loaded choices");
+        kwgpu_destroy(c2);
+    }
+    {  // PDE grouping: ten (spot, strike) pairs per PDE
+        auto base = options(60, 7);
+        std::vector<kwgpu_option> book;
+        for (const auto& b : base)
+            for (int j = 0; j < 10; ++j) {
+                kwgpu_option x = b;
+                x.s = 80 + 5 * j;
+                x.k = 115 - 2.5 * j;
+                book.push_back(x);
+            }
+        kwgpu_fd1d_params_default(&p);
+        expect(host_price(c, book, p), KWGPU_OK, c, "grouped book");
+    }
+    {  // invalid inputs
+        auto o = options(8, 9);
+        kwgpu_fd1d_params_default(&p);
+        p.x_dim = 2;
+        expect(host_price(c, o, p), KWGPU_ERR_INVALID, c, "x_dim 2");
+        kwgpu_fd1d_params_default(&p);
+        Sizes s = sizes(8, {256}, 32, 1, 0);
+        s.x[3] = 1;
+        p.x_dims = s.x.data();
+        p.t_dims = s.t.data();
+        expect(host_price(c, o, p), KWGPU_ERR_INVALID, c, "x_dims[3] = 1");
+        expect(device_price(c, o, p), KWGPU_ERR_INVALID, c, "device x_dims[3] = 1");
+    }
+    {  // surrogate: training, prediction, gated serving with per-option grids
+        kwgpu_surrogate* s = nullptr;
+        expect(kwgpu_surrogate_create(c, 2, 5, &s), KWGPU_OK, c, "surrogate create");
+        const size_t n = 600;
+        auto o = options(n, 31);
+        std::vector<double> prem(n, 0.01);
+        kwgpu_surrogate_train_params tp;
+        kwgpu_surrogate_train_params_default(&tp);
+        tp.epochs = 2;
+        tp.batch = 128;
+        std::vector<float> loss(2 * 2);
+        expect(kwgpu_surrogate_train(c, s, o.data(), prem.data(), n, &tp, loss.data()), KWGPU_OK,
+               c, "surrogate train");
+        std::vector<double> pm(n), sp(n);
+        std::vector<uint8_t> ib(n);
+        expect(kwgpu_surrogate_predict(c, s, o.data(), n, pm.data(), sp.data(), ib.data(), nullptr),
+               KWGPU_OK, c, "surrogate predict");
+        Sizes g = sizes(n, {64, 128, 256, 512}, 20, 40, 6);
+        std::vector<double> pr(n);
+        std::vector<int32_t> route(n);
+        size_t nfd = 0;
+        for (int prec : {KWGPU_FP64, KWGPU_FP32_MIXED})
+            for (double bound : {-1.0, 0.0, 1e300})
+                for (bool per_option : {false, true}) {
+                    kwgpu_fd1d_params_default(&p);
+                    p.x_dim = 128;
+                    p.t_dim = 32;
+                    p.precision = prec;
+                    if (per_option) {
+                        p.x_dims = g.x.data();
+                        p.t_dims = g.t.data();
+                    }
+                    // a smaller batch after a larger one reuses the serving buffers
+                    for (size_t m : {n, n / 3})
+                        expect(kwgpu_surrogate_price(c, s, o.data(), m, bound, &p, pr.data(),
+                                                     route.data(), &nfd),
+                               KWGPU_OK, c, "surrogate price");
+                }
+        const std::string path = dir + "/sur.bin";
+        expect(kwgpu_surrogate_save(c, s, path.c_str()), KWGPU_OK, c, "surrogate save");
+        kwgpu_surrogate* s2 = nullptr;
+        expect(kwgpu_surrogate_load(c, path.c_str(), &s2), KWGPU_OK, c, "surrogate load");
+        kwgpu_surrogate_destroy(s2);
+        kwgpu_surrogate_destroy(s);
+    }
+    kwgpu_destroy(c);
+    const int live = mock_live();
+    if (live) {
+        std::fprintf(stderr, "FAIL %d handles or allocations left alive\n", live);
+        ++failures;
+    }
+    std::printf("host logic: %s\n", failures ? "FAILED" : "ok");
+    return failures ? 1 : 0;
+}

@@ -470,6 +470,40 @@ def test_uneven_device_graph_replay(ctx, precision, monkeypatch):
     direct.close()
 
 
+def test_uneven_plans_alternate(ctx):
+    """Several uneven batches in turn on one context: each call equals a fresh context's
+    first (unplanned) call on that batch, whether it replays the cached plan (a batch
+    repeated) or re-plans (a batch alternated with another)."""
+    import torch
+    _say("uneven, alternating batches")
+    batches = []
+    for seed in range(80, 86):  # six size patterns
+        o, xd, td = W.uneven(700 + 50 * (seed - 80), seed)
+        d_o, d_x, d_t = _uneven_device(o, xd, np.minimum(td, 40))
+        n = len(o)
+        prm = kwgpu.params(x_dims=d_x.data_ptr(), t_dims=d_t.data_ptr())
+        batches.append((d_o, d_x, d_t, n, prm))
+
+    def price(c, k):
+        d_o, _, _, n, prm = batches[k]
+        d_p = torch.full((n,), -1.0, dtype=torch.float64, device="cuda")
+        d_s = torch.empty(n, dtype=torch.int32, device="cuda")
+        c.fd1d_price_device(d_o.data_ptr(), n, d_p.data_ptr(), d_s.data_ptr(), prm)
+        c.synchronize()
+        assert np.all(d_s.cpu().numpy() == 0)
+        return d_p.cpu().numpy()
+
+    first = []
+    for k in range(len(batches)):
+        fresh = kwgpu.Context(0)
+        first.append(price(fresh, k))
+        fresh.close()
+    c = kwgpu.Context(0)
+    for k in [0, 1, 0, 1, 2, 3, 0, 1, 2, 3, 4, 5, 0, 5, 4, 1]:
+        assert np.array_equal(price(c, k), first[k]), k
+    c.close()
+
+
 def test_autotune_keeps_results(ctx):
     import torch
     _say("autotune")

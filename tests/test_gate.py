@@ -30,7 +30,7 @@ def test_too_few_calibration_options_serve_nothing():
     # one tiny-spread option with a tiny error, then a large error right after it
     i = np.argsort(spread)
     err[i[1]] = 1.0
-    assert pipeline.calibrate_spread(err, spread, bar=1e-4, cap=1e-4) == 0.0
+    assert pipeline.calibrate_spread(err, spread, bar=1e-4, cap=1e-4) == -1.0
     # the same data with the count requirement waived gives the one-option bound
     assert pipeline.calibrate_spread(err, spread, bar=1e-4, cap=1e-4, min_count=1) == spread[i[0]]
 

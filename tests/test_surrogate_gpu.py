@@ -253,3 +253,44 @@ def test_serving_routes_and_fallback(ctx, tmp_path):
     assert np.array_equal(d_p2.cpu().numpy()[in_box], p[in_box])
     s.close()
     s2.close()
+
+
+def test_serving_per_option_grids(ctx):
+    """The FD fallback solves each routed option on its own grid: the compacted subset
+    carries its options' sizes, as the uneven-batch pricer takes them."""
+    import torch
+    _say("serving, per-option grids")
+    o = W.serving(3000, 43)
+    n = len(o)
+    rng = np.random.default_rng(43)
+    xd = rng.choice(np.array([64, 128, 256, 512], np.int32), n)
+    td = np.minimum(W.t_points_for(o["t"]), 160).astype(np.int32)
+    d_o = torch.from_numpy(o.view(np.uint8).copy()).cuda()
+    d_x = torch.from_numpy(xd.copy()).cuda()
+    d_t = torch.from_numpy(td.copy()).cuda()
+    s = kwgpu.Surrogate(ctx, members=2, seed=5)
+    d_p = torch.empty(n, dtype=torch.float64, device="cuda")
+    d_r = torch.empty(n, dtype=torch.int32, device="cuda")
+    ref, st, rc, _ = ctx.fd1d_price(o, 512, 512, x_dims=xd, t_dims=td, precision=kwgpu.FP64)
+    assert rc == 0 and np.all(st == 0)
+    # a uniform grid of any of the sizes differs from the per-option prices
+    uni, _, _, _ = ctx.fd1d_price(o, 256, 160, precision=kwgpu.FP64)
+    assert np.max(np.abs(uni - ref)) > 1e-3
+    for prec, tol in ((kwgpu.FP64, 1e-8), (kwgpu.FP32_MIXED, 1e-4)):
+        fd = kwgpu.params(512, 512, precision=prec, x_dims=d_x.data_ptr(), t_dims=d_t.data_ptr())
+        # every option to FD
+        assert s.price(d_o.data_ptr(), n, -1.0, fd, d_p.data_ptr(), d_r.data_ptr()) == n
+        assert np.max(np.abs(d_p.cpu().numpy() - ref)) <= tol
+        # only the out-of-box options to FD: a scattered subset keeps its own sizes
+        feats = kwgpu.surrogate_features(o)
+        out = ~np.all(np.abs(feats) <= 1 + 1e-9, axis=1)
+        assert 0 < out.sum() < n
+        nfd = s.price(d_o.data_ptr(), n, 1e300, fd, d_p.data_ptr(), d_r.data_ptr())
+        assert nfd == int(out.sum())
+        assert np.array_equal(d_r.cpu().numpy() != 0, out)
+        p1 = d_p.cpu().numpy()
+        assert np.max(np.abs(p1[out] - ref[out])) <= tol
+        # the same routing gives the same prices again (the subset is compacted in order)
+        assert s.price(d_o.data_ptr(), n, 1e300, fd, d_p.data_ptr(), d_r.data_ptr()) == nfd
+        assert np.array_equal(d_p.cpu().numpy(), p1, equal_nan=True)
+    s.close()

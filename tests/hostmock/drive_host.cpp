@@ -114,6 +114,14 @@ int main(int argc, char** argv) {
         p.x_dims = s2.x.data();
         p.t_dims = s2.t.data();
         expect(host_price(c, o, p), KWGPU_OK, c, "mixed uneven");
+        // FP32-mixed with PDEs that go to FP64 (scale sigma sqrt T > 80): two solves
+        for (size_t i = 0; i < o.size(); i += 3) o[i].z = 1.5, o[i].t = 10.0;
+        expect(host_price(c, o, p), KWGPU_OK, c, "mixed uneven with FP64 PDEs");
+        kwgpu_fd1d_params_default(&p);
+        p.precision = KWGPU_FP32_MIXED;
+        expect(host_price(c, o, p), KWGPU_OK, c, "mixed uniform with FP64 PDEs");
+        for (auto& x : o) x.z = 2.0, x.t = 20.0;
+        expect(host_price(c, o, p), KWGPU_OK, c, "mixed, every PDE in FP64");
     }
     for (int x : {3, 17, 64, 100, 129, 256, 300, 511, 1000, 1024, 2048, 4096})  // uniform grids
         for (int lay : {KWGPU_LAYOUT_AUTO, KWGPU_LAYOUT_WAVE_REG, KWGPU_LAYOUT_LANE_SOA}) {

@@ -440,10 +440,12 @@ def test_mixed_sizes_vs_oracle(ctx, x_dim, t_dim):
 
 def test_mixed_validated_domain(ctx):
     """FP32-mixed over a wide random box at 1024 x 1024 (vol 2 %-120 %, maturity
-    0.002-15 years, rates -3 %-20 %, yields 0-25 %, spot 0.4-2.5x strike): within 1e-4
-    relative of FP64 wherever scale sigma sqrt(T) < 80 (measured: at most 3.2e-8); above
-    that, calls whose FP32 increments overflow are flagged non-finite, and no price is
-    silently outside the bar."""
+    0.002-15 years, rates -3 %-20 %, yields 0-25 %, spot 0.4-2.5x strike). Device entry
+    point (one solve per option, as asked): within 1e-4 relative of FP64 wherever
+    scale sigma sqrt(T) < 80 (measured: at most 3.2e-8); above that, calls whose FP32
+    increments overflow are flagged non-finite, and no price is silently outside the bar.
+    Host entry point: those PDEs are solved in FP64, so every option is priced."""
+    import torch
     _say("mixed validated domain")
     rng = np.random.default_rng(7)
     n = 16384
@@ -457,21 +459,33 @@ def test_mixed_validated_domain(ctx):
     o["w"] = np.where(rng.random(n) < 0.5, -1, 1)
     o["e"] = 1
     p64, s64, _, _ = ctx.fd1d_price(o, 1024, 1024)
-    p32, s32, _, _ = ctx.fd1d_price(o, 1024, 1024, precision=kwgpu.FP32_MIXED)
     ok64 = (s64 == kwgpu.STATUS_OK) & np.isfinite(p64)
     assert ok64.mean() > 0.9
+    inside = ok64 & (50.0 * o["z"] * np.sqrt(o["t"]) < 80.0)
+    over = ok64 & ~inside
+
+    d_o = torch.from_numpy(o.view(np.uint8).copy()).cuda()
+    d_p = torch.empty(n, dtype=torch.float64, device="cuda")
+    d_s = torch.empty(n, dtype=torch.int32, device="cuda")
+    ctx.fd1d_price_device(d_o.data_ptr(), n, d_p.data_ptr(), d_s.data_ptr(),
+                          kwgpu.params(1024, 1024, precision=kwgpu.FP32_MIXED))
+    ctx.synchronize()
+    p32, s32 = d_p.cpu().numpy(), d_s.cpu().numpy()
     ok32 = (s32 == kwgpu.STATUS_OK) & np.isfinite(p32)
     rel = np.abs(p32 - p64) / np.maximum(1.0, np.abs(p64))
-    inside = ok64 & (50.0 * o["z"] * np.sqrt(o["t"]) < 80.0)
     assert np.all(ok32[inside])
     _say("   inside", int(inside.sum()), "max rel", float(rel[inside].max()))
     assert rel[inside].max() <= 1e-4
-    # beyond it the calls overflow and say so; nothing anywhere is silently wrong
-    over = ok64 & ~inside
     flagged = over & ~ok32
     assert flagged.any() and np.all(o["w"][flagged] == 1)
     assert np.all(s32[flagged] == kwgpu.STATUS_NONFINITE)
     assert np.all(rel[ok64 & ok32] <= 1e-4)
+
+    h32, hs32, rc, err = ctx.fd1d_price(o, 1024, 1024, precision=kwgpu.FP32_MIXED)
+    assert np.all(hs32[ok64] == kwgpu.STATUS_OK)
+    hrel = np.abs(h32 - p64) / np.maximum(1.0, np.abs(p64))
+    assert hrel[inside].max() <= 1e-4
+    assert np.all(hrel[over] <= EPS)  # the FP64 form itself
 
 
 def test_mixed_full_size_per_class(ctx):

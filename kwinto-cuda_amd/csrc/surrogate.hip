@@ -388,7 +388,8 @@ __global__ void out_bwd(const float* Z4, long sz, const float* p, const float* d
 // grad[kB? + n] += sum_m dZ[m][n]   and, for the output layer,
 // grad[kW5 + n] += sum_m dy[m] silu(Z4[m][n]), grad[kB5] += sum_m dy[m]
 // The column reductions below: 256 threads = 64 columns x 4 row phases; the four
-// phases are summed in LDS, so each column takes one atomic per block.
+// phases are summed in LDS, and each block stores one partial sum per column into its
+// own slot of `part` (added up by sum_parts).
 __device__ __forceinline__ float phase_sum(float v, float* lds) {
     lds[threadIdx.x] = v;
     __syncthreads();
@@ -432,7 +433,7 @@ __global__ __launch_bounds__(256) void out_grad(const float* Z4, long sz, const 
 // Layer-1 gradient: seven sums per hidden column (six W1 weights and the bias). Each
 // lane owns four adjacent columns, so a wave reads one whole 512-byte dZ1 row per
 // iteration and the row's six inputs once (wave-uniform); the four waves' partial sums
-// meet in LDS and each block adds 1792 values with one atomic each.
+// meet in LDS and each block stores its 1792 sums into its own slot of `part`.
 typedef bf16 bf16x4 __attribute__((ext_vector_type(4)));
 __global__ __launch_bounds__(256) void l1_grad(const bf16* dZ1, long sdz, const float* x, long sx,
                                                int n, float* part) {

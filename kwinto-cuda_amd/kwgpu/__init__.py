@@ -327,6 +327,10 @@ class Surrogate:
 
     def price(self, d_opts: int, n: int, spread_max: float, fd: Fd1dParams, d_prices: int,
               d_route: int) -> int:
+        """Surrogate-first prices into d_prices; d_route[i] = 0 (surrogate), 1 (FD at
+        fd.precision) or 2 (FD in FP64). A negative spread_max serves nothing and skips
+        the ensemble. fd.x_dims / fd.t_dims may hold device arrays of per-option grid
+        sizes. Returns the number of options priced by FD."""
         nfd = ctypes.c_size_t(0)
         self._check(lib().kwgpu_surrogate_price(self.ctx.h, self.h, d_opts, n, spread_max,
                                                 ctypes.byref(fd), d_prices, d_route,

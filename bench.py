@@ -8,8 +8,10 @@ grid, FP64, through kwgpu_fd1d_price_device (one kernel launch per step).
 Prints one JSON line. `value` is options/s of the whole batch; `roofline` prices the
 kernel's algorithmic FP64 work (10 flops per point and step) against the FP64 peak
 using the kernel duration measured with HIP events on the launch stream;
-`cpu_baseline` times the CPU restatement of the reference pricer (its thread-pool
-path, one job per PDE) on a bounded sample of the same workload.
+`cpu_baseline` times the reference pricer itself (its sources compiled into
+oracle/_ref, its thread pool on every hardware thread) on a bounded sample of the same
+workload, with the oracle's restatement of it timed beside (or alone, where the
+reference library was not built).
 
     python bench.py [--gpus 1] [--steps 10] [--warmup 2]
 """
@@ -122,22 +124,42 @@ def cpu_model():
 
 
 def cpu_baseline(opts, sample, threads):
+    """The reference pricer itself (its sources compiled into oracle/_ref, its
+    ThreadPool on every hardware thread) where that library was built, else the
+    oracle's restatement of it on `threads` threads; the other one is timed beside it."""
     import oracle
     sub = opts[:sample]
-    oracle.fd1d_price(sub[:min(16, sample)], X_DIM, T_DIM, nthreads=threads)  # warm-up
-    runs = []
-    for _ in range(3):  # the median of three timed runs
-        t0 = time.perf_counter()
-        _, err = oracle.fd1d_price(sub, X_DIM, T_DIM, nthreads=threads)
-        runs.append(time.perf_counter() - t0)
-        if err:
-            raise RuntimeError(err)
-    dt = float(np.median(runs))
-    return {"value": sample / dt, "unit": "options/s", "cores": threads, "kind": "port",
-            "sample": f"first {sample} options of the same workload, {X_DIM}x{T_DIM} FP64, "
-                      f"{threads} threads, one job per PDE, median of 3 runs "
-                      f"({', '.join(f'{r:.2f}' for r in runs)} s); host {cpu_model()}, "
-                      f"{os.cpu_count()} logical CPUs visible"}
+
+    def timed(price):
+        price(sub[:min(16, sample)])  # warm-up
+        runs = []
+        for _ in range(3):  # the median of three timed runs
+            t0 = time.perf_counter()
+            _, err = price(sub)
+            runs.append(time.perf_counter() - t0)
+            if err:
+                raise RuntimeError(err)
+        return float(np.median(runs)), runs
+
+    port = lambda s: oracle.fd1d_price(s, X_DIM, T_DIM, nthreads=threads)
+    legs = [("port", port, threads, "the oracle's restatement, one job per PDE")]
+    if oracle.ref_available():
+        ref = lambda s: oracle.ref_price("FD1D", s, X_DIM, T_DIM)
+        legs.insert(0, ("reference", ref, oracle.ref_threads(),
+                        "the reference's Fd1d_Pricer and ThreadPool, compiled from its sources"))
+    out = None
+    for kind, price, cores, what in legs:
+        dt, runs = timed(price)
+        leg = {"value": sample / dt, "unit": "options/s", "cores": cores, "kind": kind,
+               "sample": f"first {sample} options of the same workload, {X_DIM}x{T_DIM} FP64, "
+                         f"{what}, {cores} threads, median of 3 runs "
+                         f"({', '.join(f'{r:.2f}' for r in runs)} s); host {cpu_model()}, "
+                         f"{os.cpu_count()} logical CPUs visible"}
+        if out is None:
+            out = leg
+        else:
+            out["beside"] = leg
+    return out
 
 
 def bench_uneven(ctx, torch, stream, n, steps, log):

@@ -70,6 +70,7 @@ def ref_lib():
         R = ctypes.CDLL(REF_LIB_PATH)
         R.kwref_price.argtypes = [ctypes.c_char_p, _P, _SZ, _D, _D, ctypes.c_longlong,
                                   ctypes.c_longlong, _P, ctypes.c_char_p, _SZ]
+        R.kwref_threads.restype = ctypes.c_int
         _ref = R
     return _ref
 
@@ -141,6 +142,11 @@ def solve_tridiagonal(al, a, au, y):
     x = np.zeros(n)
     rc = lib().kwo_solve_tridiagonal(n, *[v.ctypes.data for v in arrs], x.ctypes.data)
     return x, rc
+
+
+def ref_threads() -> int:
+    """Worker threads the compiled reference's thread pool runs on this host."""
+    return int(ref_lib().kwref_threads())
 
 
 def ref_price(pricer, opts, x_dim=512, t_dim=512, density=0.25, scale=50.0):

@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "Pricer/kwPricerFactory.h"
@@ -38,3 +39,7 @@ extern "C" int kwref_price(const char* pricer, const void* opts, size_t n, doubl
     }
     return 0;
 }
+
+// Worker threads of the reference's ThreadPool (src/kwThreadPool.cpp:9-16): one per
+// hardware thread.
+extern "C" int kwref_threads() { return (int)std::thread::hardware_concurrency(); }

@@ -38,6 +38,7 @@ struct Bucket {
     Choice ch;
     size_t begin, count;
     int max_x;
+    int prec = KWGPU_FP64;  // the march's precision for this bucket's PDEs
 };
 
 constexpr int kLaneCap = 1 << 30;
@@ -59,6 +60,7 @@ struct kwgpu_ctx {
     bool plan_valid = false;
     uint64_t plan_key = 0;              // hash of everything but the sizes
     std::vector<int32_t> plan_x, plan_t;  // the sizes the plan was made for (compared exactly)
+    std::vector<uint8_t> plan_p;          // ... and the per-PDE precisions (empty: one for all)
     int plan_prec = -1, plan_layout = -1;  // ... and the precision and layout it bucketed for
     int32_t* hsizes = nullptr;          // pinned staging for device-resident sizes
     size_t hsizes_cap = 0;
@@ -175,7 +177,7 @@ uint64_t fnv_batch(uint64_t h, const FdBatch& b) {
 
 uint64_t fnv_bucket(uint64_t h, const Bucket& k) {
     const int persist = k.ch.persist ? 1 : 0;
-    return fnv_fields(h, k.cap, k.ch.variant, persist, k.begin, k.count, k.max_x);
+    return fnv_fields(h, k.cap, k.ch.variant, persist, k.begin, k.count, k.max_x, k.prec);
 }
 
 int lane_launch(kwgpu_ctx* c, FdBatch bb, int max_x, int precision, hipStream_t s, double* ws) {
@@ -192,7 +194,7 @@ int lane_launch(kwgpu_ctx* c, FdBatch bb, int max_x, int precision, hipStream_t 
 // Launches the buckets of an uneven batch: in order on the caller's stream when
 // `serial`, else bucket i on side stream i % kSide (forked
 // from and joined back into s), persistent buckets taking work from counters[i].
-int launch_buckets(kwgpu_ctx* c, const FdBatch& b, const std::vector<Bucket>& plan, int precision,
+int launch_buckets(kwgpu_ctx* c, const FdBatch& b, const std::vector<Bucket>& plan,
                    hipStream_t s, bool serial) {
     hipError_t e;
     if ((e = kwgpu::launch_zero_counters((int*)c->counters.p, (int)plan.size(), s)))
@@ -212,8 +214,8 @@ int launch_buckets(kwgpu_ctx* c, const FdBatch& b, const std::vector<Bucket>& pl
         if (k.ch.variant < 0) {
             double* ws = (double*)c->ws.p + ws_off;
             ws_off += kwgpu::lane_workspace_doubles(bb.m, k.max_x);
-            if (int rc = lane_launch(c, bb, k.max_x, precision, ss, ws)) return rc;
-        } else if ((e = kwgpu::launch_fd1d_wave_variant(bb, k.ch.variant, precision, ss,
+            if (int rc = lane_launch(c, bb, k.max_x, k.prec, ss, ws)) return rc;
+        } else if ((e = kwgpu::launch_fd1d_wave_variant(bb, k.ch.variant, k.prec, ss,
                                                         k.ch.persist ? (int*)c->counters.p + i
                                                                      : nullptr))) {
             return hip_fail(c, e, "kwgpu: wave kernel launch");
@@ -247,7 +249,8 @@ void drop_graph(kwgpu_ctx* c) {
 
 // Runs the FD solve of `b` (device buffers already set) on stream s. pde_x / pde_t are
 // host copies of each PDE's grid sizes (b.m each); null means every PDE uses b.x_dim /
-// b.t_dim.
+// b.t_dim. pde_prec (host, b.m, optional) gives each PDE its own precision; the buckets
+// then split by precision too, and the batch is still one plan and one graph.
 //
 // Uneven batches: PDEs are grouped by grid-size bucket, each bucket sorted by t-points
 // (descending) so that co-packed segments march for about the same number of steps and
@@ -255,24 +258,44 @@ void drop_graph(kwgpu_ctx* c) {
 // runs on its own side stream. The whole launch sequence is captured once into a
 // hipGraph and replayed while the batch (sizes, buffers, parameters) is unchanged.
 int run_fd(kwgpu_ctx* c, FdBatch b, const int32_t* pde_x, const int32_t* pde_t, int precision,
-           int layout, hipStream_t s) {
+           int layout, hipStream_t s, const uint8_t* pde_prec = nullptr) {
+    // per-PDE precisions go through the planned path, with the uniform sizes spelled out
+    std::vector<int32_t> ux, ut;
+    if (pde_prec && !pde_x) {
+        ux.assign((size_t)b.m, b.x_dim);
+        ut.assign((size_t)b.m, b.t_dim);
+        pde_x = ux.data();
+        pde_t = ut.data();
+    }
     const bool uniform = pde_x == nullptr;
-    if (precision == KWGPU_FP32_MIXED && layout == KWGPU_LAYOUT_LANE_SOA)
-        return fail(c, KWGPU_ERR_UNSUPPORTED, "kwgpu: FP32-mixed runs in the wave layout only");
-    // a batch with exactly the cached plan's sizes, precision and layout has the plan's
-    // buckets: checked first, so a replay does no per-PDE pass but these two compares
+    auto prec_of = [&](int p) { return pde_prec ? (int)pde_prec[p] : precision; };
+    // a batch with exactly the cached plan's sizes, precisions and layout has the plan's
+    // buckets: checked first, so a replay does no per-PDE pass but these compares
     const size_t bytes = sizeof(int32_t) * (size_t)b.m;
-    const bool same_sizes = !uniform && c->plan_valid && c->plan_prec == precision &&
-                            c->plan_layout == layout && c->plan_x.size() == (size_t)b.m &&
-                            !std::memcmp(c->plan_x.data(), pde_x, bytes) &&
-                            !std::memcmp(c->plan_t.data(), pde_t, bytes);
-    const int mx = uniform      ? b.x_dim
-                   : same_sizes ? c->plan.back().max_x
-                                : *std::max_element(pde_x, pde_x + b.m);
-    if (layout == KWGPU_LAYOUT_WAVE_REG && !kwgpu::wave_layout_fits(mx, precision))
-        return fail(c, KWGPU_ERR_UNSUPPORTED, "kwgpu: wave layout supports x_dim <= 2048");
-    if (precision == KWGPU_FP32_MIXED && !kwgpu::wave_layout_fits(mx, precision))
-        return fail(c, KWGPU_ERR_UNSUPPORTED, "kwgpu: FP32-mixed supports x_dim <= 2048");
+    const bool same_sizes =
+        !uniform && c->plan_valid && c->plan_prec == precision && c->plan_layout == layout &&
+        c->plan_x.size() == (size_t)b.m && !std::memcmp(c->plan_x.data(), pde_x, bytes) &&
+        !std::memcmp(c->plan_t.data(), pde_t, bytes) &&
+        (pde_prec ? c->plan_p.size() == (size_t)b.m && !std::memcmp(c->plan_p.data(), pde_prec, b.m)
+                  : c->plan_p.empty());
+    if (!same_sizes) {  // (a replayed plan passed these checks when it was made)
+        int mx = b.x_dim, mx_mixed = 0;
+        if (!uniform) {
+            mx = 0;
+            for (int p = 0; p < b.m; ++p) {
+                mx = std::max(mx, pde_x[p]);
+                if (prec_of(p) == KWGPU_FP32_MIXED) mx_mixed = std::max(mx_mixed, pde_x[p]);
+            }
+        } else if (precision == KWGPU_FP32_MIXED) {
+            mx_mixed = b.x_dim;
+        }
+        if (mx_mixed && layout == KWGPU_LAYOUT_LANE_SOA)
+            return fail(c, KWGPU_ERR_UNSUPPORTED, "kwgpu: FP32-mixed runs in the wave layout only");
+        if (layout == KWGPU_LAYOUT_WAVE_REG && !kwgpu::wave_layout_fits(mx, precision))
+            return fail(c, KWGPU_ERR_UNSUPPORTED, "kwgpu: wave layout supports x_dim <= 2048");
+        if (mx_mixed && !kwgpu::wave_layout_fits(mx_mixed, KWGPU_FP32_MIXED))
+            return fail(c, KWGPU_ERR_UNSUPPORTED, "kwgpu: FP32-mixed supports x_dim <= 2048");
+    }
 
     if (uniform) {
         const int cap = bucket_of(b.x_dim, precision, layout);
@@ -291,30 +314,33 @@ int run_fd(kwgpu_ctx* c, FdBatch b, const int32_t* pde_x, const int32_t* pde_t, 
     }
 
     const int m = b.m;
-    // buckets in ascending capacity (64 .. 2048, then the lane layout): counts and widest
-    // grid, without sorting the PDEs
+    // buckets in ascending capacity (64 .. 2048, then the lane layout), FP64 ones first,
+    // then FP32-mixed ones: counts and widest grid, without sorting the PDEs
     constexpr int kSlots = 7;
-    auto slot_of = [&](int x) {
-        const int cap = bucket_of(x, precision, layout);
-        return cap == kLaneCap ? kSlots - 1 : 31 - __builtin_clz((unsigned)cap) - 6;
+    auto slot_of = [&](int p) {
+        const int cap = bucket_of(pde_x[p], precision, layout);
+        const int k = cap == kLaneCap ? kSlots - 1 : 31 - __builtin_clz((unsigned)cap) - 6;
+        return prec_of(p) == KWGPU_FP32_MIXED ? kSlots + k : k;
     };
     std::vector<Bucket> plan;
     if (same_sizes) {
         plan = c->plan;
-        for (Bucket& k : plan) k.ch = choice_for(c, k.cap, precision, layout);
+        for (Bucket& k : plan) k.ch = choice_for(c, k.cap, k.prec, layout);
     } else {
-        size_t count[kSlots] = {};
-        int widest[kSlots] = {};
+        size_t count[2 * kSlots] = {};
+        int widest[2 * kSlots] = {};
         for (int p = 0; p < m; ++p) {
-            const int k = slot_of(pde_x[p]);
+            const int k = slot_of(p);
             ++count[k];
             widest[k] = std::max(widest[k], pde_x[p]);
         }
         size_t begin = 0;
-        for (int k = 0; k < kSlots; ++k) {
+        for (int k = 0; k < 2 * kSlots; ++k) {
             if (!count[k]) continue;
-            const int cap = k == kSlots - 1 ? kLaneCap : 64 << k;
-            plan.push_back(Bucket{cap, choice_for(c, cap, precision, layout), begin, count[k], widest[k]});
+            const int kc = k % kSlots;
+            const int cap = kc == kSlots - 1 ? kLaneCap : 64 << kc;
+            const int pr = k < kSlots ? KWGPU_FP64 : KWGPU_FP32_MIXED;
+            plan.push_back(Bucket{cap, choice_for(c, cap, pr, layout), begin, count[k], widest[k], pr});
             begin += count[k];
         }
     }
@@ -347,7 +373,7 @@ int run_fd(kwgpu_ctx* c, FdBatch b, const int32_t* pde_x, const int32_t* pde_t, 
         key = with_buffers(key);
         // PDEs by bucket, longest march first within a bucket (ties by index)
         std::vector<int> slot(m);
-        for (int p = 0; p < m; ++p) slot[p] = slot_of(pde_x[p]);
+        for (int p = 0; p < m; ++p) slot[p] = slot_of(p);
         std::vector<uint32_t> ord(m);
         std::iota(ord.begin(), ord.end(), 0u);
         std::sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t z) {
@@ -361,16 +387,20 @@ int run_fd(kwgpu_ctx* c, FdBatch b, const int32_t* pde_x, const int32_t* pde_t, 
         c->plan_key = key;
         c->plan_x.assign(pde_x, pde_x + m);
         c->plan_t.assign(pde_t, pde_t + m);
+        if (pde_prec)
+            c->plan_p.assign(pde_prec, pde_prec + m);
+        else
+            c->plan_p.clear();
         c->plan_prec = precision;
         c->plan_layout = layout;
         c->plan_valid = true;
     }
-    if (!c->use_graphs) return launch_buckets(c, b, c->plan, precision, s, serial);
+    if (!c->use_graphs) return launch_buckets(c, b, c->plan, s, serial);
     if (!c->graph) {
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
         if (e == hipSuccess) {
-            const int rc = launch_buckets(c, b, c->plan, precision, s, serial);
+            const int rc = launch_buckets(c, b, c->plan, s, serial);
             hipError_t e2 = hipStreamEndCapture(s, &g);
             if (rc) {
                 if (g) (void)hipGraphDestroy(g);
@@ -384,7 +414,7 @@ int run_fd(kwgpu_ctx* c, FdBatch b, const int32_t* pde_x, const int32_t* pde_t, 
             // streams that cannot be captured (e.g. the legacy null stream): launch directly
             (void)hipGetLastError();
             c->graph = nullptr;
-            return launch_buckets(c, b, c->plan, precision, s, serial);
+            return launch_buckets(c, b, c->plan, s, serial);
         }
     }
     if (hipError_t e = hipGraphLaunch(c->graph, s); e != hipSuccess)
@@ -474,29 +504,18 @@ int fd_price_host(kwgpu_ctx* c, const kwgpu_option* opts, size_t n, const kwgpu_
     const size_t m = rep.size();
     // FP32-mixed: a PDE whose FP32 increments could overflow (the far call payoff
     // e^(scale sigma sqrt T) beyond the FP32 range) is solved in FP64, as the serving
-    // gate does. Those PDEs go last, and the batch runs as two solves.
-    size_t m64 = 0;
+    // gate does, in the same solve
+    std::vector<uint8_t> pprec;
     if (p->precision == KWGPU_FP32_MIXED) {
-        std::vector<uint32_t> ord, late;
+        bool any = false;
+        pprec.resize(m);
         for (size_t k = 0; k < m; ++k) {
             const kwgpu_option& a = opts[rep[k]];
-            (p->scale * a.z * std::sqrt(a.t) > 80.0 ? late : ord).push_back((uint32_t)k);
+            const bool f64 = p->scale * a.z * std::sqrt(a.t) > 80.0;
+            pprec[k] = f64 ? KWGPU_FP64 : KWGPU_FP32_MIXED;
+            any = any || f64;
         }
-        m64 = late.size();
-        if (m64) {
-            ord.insert(ord.end(), late.begin(), late.end());
-            std::vector<uint32_t> rep2(m), off2(m + 1), list2;
-            list2.reserve(n);
-            off2[0] = 0;
-            for (size_t k = 0; k < m; ++k) {
-                rep2[k] = rep[ord[k]];
-                list2.insert(list2.end(), list.begin() + off[ord[k]], list.begin() + off[ord[k] + 1]);
-                off2[k + 1] = (uint32_t)list2.size();
-            }
-            rep.swap(rep2);
-            off.swap(off2);
-            list.swap(list2);
-        }
+        if (!any) pprec.clear();
     }
 
     hipError_t e;
@@ -552,21 +571,9 @@ int fd_price_host(kwgpu_ctx* c, const kwgpu_option* opts, size_t n, const kwgpu_
         // the host vectors must outlive the async copies
         if ((e = hipStreamSynchronize(s)) != hipSuccess) return hip_fail(c, e, "kwgpu: sync");
     }
-    const size_t m32 = m - m64;
-    int rc = KWGPU_OK;
-    if (m32) {
-        b.m = (int)m32;
-        rc = run_fd(c, b, pde_x.empty() ? nullptr : pde_x.data(),
-                    pde_t.empty() ? nullptr : pde_t.data(), p->precision, p->layout, s);
-    }
-    if (rc == KWGPU_OK && m64) {
-        FdBatch b64 = b;
-        b64.rep += m32;
-        b64.off += m32;
-        b64.m = (int)m64;
-        rc = run_fd(c, b64, pde_x.empty() ? nullptr : pde_x.data() + m32,
-                    pde_t.empty() ? nullptr : pde_t.data() + m32, KWGPU_FP64, p->layout, s);
-    }
+    const int rc = run_fd(c, b, pde_x.empty() ? nullptr : pde_x.data(),
+                          pde_t.empty() ? nullptr : pde_t.data(), p->precision, p->layout, s,
+                          pprec.empty() ? nullptr : pprec.data());
     // the host vectors above must outlive every copy queued from them
     e = hipStreamSynchronize(s);
     if (rc) return rc;
@@ -616,6 +623,59 @@ namespace kwgpu {
 int ctx_fail(kwgpu_ctx* c, int code, const std::string& msg) { return fail(c, code, msg); }
 hipStream_t ctx_stream(kwgpu_ctx* c) { return c ? c->stream : nullptr; }
 int ctx_device(kwgpu_ctx* c) { return c ? c->device : 0; }
+
+// kwgpu_fd1d_price_device with an optional precision per option (host array of n,
+// KWGPU_FP64 or KWGPU_FP32_MIXED; null: params->precision for all): one plan, one graph.
+int fd_price_device_prec(kwgpu_ctx* c, const kwgpu_option* d_opts, size_t n,
+                         const kwgpu_fd1d_params* p, double* d_prices, int32_t* d_status,
+                         void* stream, const uint8_t* host_prec) {
+    if (!c) return KWGPU_ERR_INVALID;
+    c->err.clear();
+    if (int rc = check_params(c, p)) return rc;
+    if (n == 0) return KWGPU_OK;
+    if (!d_opts || !d_prices) return fail(c, KWGPU_ERR_INVALID, "kwgpu: null pointer");
+    if (n > 0x7fffffffu) return fail(c, KWGPU_ERR_INVALID, "kwgpu: too many options");
+    if (hipError_t e = hipSetDevice(c->device); e != hipSuccess) return hip_fail(c, e, "hipSetDevice");
+    hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+    FdBatch b{};
+    b.opts = d_opts;
+    b.m = (int)n;
+    b.x_dim = p->x_dim;
+    b.t_dim = p->t_dim;
+    b.x_dims = p->x_dims;
+    b.t_dims = p->t_dims;
+    b.density = p->density;
+    b.scale = p->scale;
+    b.theta = p->theta;
+    b.prices = d_prices;
+    b.status = d_status;
+    if (!p->x_dims && !p->t_dims)
+        return run_fd(c, b, nullptr, nullptr, p->precision, p->layout, s, host_prec);
+    // grouping needs the sizes on the host: pinned staging owned by the context
+    if (c->hsizes_cap < 2 * n) {
+        if (c->hsizes) (void)hipHostFree(c->hsizes);
+        c->hsizes = nullptr;
+        c->hsizes_cap = 0;
+        if (hipHostMalloc((void**)&c->hsizes, 2 * n * sizeof(int32_t), hipHostMallocDefault) !=
+            hipSuccess)
+            return fail(c, KWGPU_ERR_NOMEM, "kwgpu: pinned staging allocation failed");
+        c->hsizes_cap = 2 * n;
+    }
+    int32_t* hx = c->hsizes;
+    int32_t* ht = c->hsizes + n;
+    hipError_t e;
+    if (!p->x_dims) std::fill(hx, hx + n, p->x_dim);
+    if (!p->t_dims) std::fill(ht, ht + n, p->t_dim);
+    if (p->x_dims &&
+        (e = hipMemcpyAsync(hx, p->x_dims, n * sizeof(int32_t), hipMemcpyDeviceToHost, s)))
+        return hip_fail(c, e, "kwgpu: x_dims download");
+    if (p->t_dims &&
+        (e = hipMemcpyAsync(ht, p->t_dims, n * sizeof(int32_t), hipMemcpyDeviceToHost, s)))
+        return hip_fail(c, e, "kwgpu: t_dims download");
+    if ((e = hipStreamSynchronize(s)) != hipSuccess) return hip_fail(c, e, "kwgpu: sync");
+    if (int rc = check_dims(c, hx, ht, n)) return rc;
+    return run_fd(c, b, hx, ht, p->precision, p->layout, s, host_prec);
+}
 }  // namespace kwgpu
 
 extern "C" {
@@ -702,51 +762,7 @@ int kwgpu_fd1d_price(kwgpu_ctx* c, const kwgpu_option* opts, size_t n,
 int kwgpu_fd1d_price_device(kwgpu_ctx* c, const kwgpu_option* d_opts, size_t n,
                             const kwgpu_fd1d_params* p, double* d_prices, int32_t* d_status,
                             void* stream) {
-    if (!c) return KWGPU_ERR_INVALID;
-    c->err.clear();
-    if (int rc = check_params(c, p)) return rc;
-    if (n == 0) return KWGPU_OK;
-    if (!d_opts || !d_prices) return fail(c, KWGPU_ERR_INVALID, "kwgpu: null pointer");
-    if (n > 0x7fffffffu) return fail(c, KWGPU_ERR_INVALID, "kwgpu: too many options");
-    if (hipError_t e = hipSetDevice(c->device); e != hipSuccess) return hip_fail(c, e, "hipSetDevice");
-    hipStream_t s = stream ? (hipStream_t)stream : c->stream;
-    FdBatch b{};
-    b.opts = d_opts;
-    b.m = (int)n;
-    b.x_dim = p->x_dim;
-    b.t_dim = p->t_dim;
-    b.x_dims = p->x_dims;
-    b.t_dims = p->t_dims;
-    b.density = p->density;
-    b.scale = p->scale;
-    b.theta = p->theta;
-    b.prices = d_prices;
-    b.status = d_status;
-    if (!p->x_dims && !p->t_dims) return run_fd(c, b, nullptr, nullptr, p->precision, p->layout, s);
-    // grouping needs the sizes on the host: pinned staging owned by the context
-    if (c->hsizes_cap < 2 * n) {
-        if (c->hsizes) (void)hipHostFree(c->hsizes);
-        c->hsizes = nullptr;
-        c->hsizes_cap = 0;
-        if (hipHostMalloc((void**)&c->hsizes, 2 * n * sizeof(int32_t), hipHostMallocDefault) !=
-            hipSuccess)
-            return fail(c, KWGPU_ERR_NOMEM, "kwgpu: pinned staging allocation failed");
-        c->hsizes_cap = 2 * n;
-    }
-    int32_t* hx = c->hsizes;
-    int32_t* ht = c->hsizes + n;
-    hipError_t e;
-    if (!p->x_dims) std::fill(hx, hx + n, p->x_dim);
-    if (!p->t_dims) std::fill(ht, ht + n, p->t_dim);
-    if (p->x_dims &&
-        (e = hipMemcpyAsync(hx, p->x_dims, n * sizeof(int32_t), hipMemcpyDeviceToHost, s)))
-        return hip_fail(c, e, "kwgpu: x_dims download");
-    if (p->t_dims &&
-        (e = hipMemcpyAsync(ht, p->t_dims, n * sizeof(int32_t), hipMemcpyDeviceToHost, s)))
-        return hip_fail(c, e, "kwgpu: t_dims download");
-    if ((e = hipStreamSynchronize(s)) != hipSuccess) return hip_fail(c, e, "kwgpu: sync");
-    if (int rc = check_dims(c, hx, ht, n)) return rc;
-    return run_fd(c, b, hx, ht, p->precision, p->layout, s);
+    return kwgpu::fd_price_device_prec(c, d_opts, n, p, d_prices, d_status, stream, nullptr);
 }
 
 int kwgpu_bs_price(kwgpu_ctx* c, const kwgpu_option* opts, size_t n, double* prices) {

@@ -158,6 +158,10 @@ namespace kwgpu {
 // Context internals shared by the C-ABI translation units (kwgpu_api.hip).
 int ctx_fail(kwgpu_ctx* c, int code, const std::string& msg);
 hipStream_t ctx_stream(kwgpu_ctx* c);
+// kwgpu_fd1d_price_device with an optional per-option precision (host array of n).
+int fd_price_device_prec(kwgpu_ctx* c, const kwgpu_option* d_opts, size_t n,
+                         const kwgpu_fd1d_params* p, double* d_prices, int32_t* d_status,
+                         void* stream, const uint8_t* host_prec);
 int ctx_device(kwgpu_ctx* c);
 
 // Wave layout, FP64 / FP32-mixed. Returns hipErrorInvalidValue when x_dim does not fit.

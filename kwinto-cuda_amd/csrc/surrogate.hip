@@ -651,10 +651,11 @@ __global__ __launch_bounds__(1024) void scan_block_counts(uint32_t* blk_counts, 
     }
 }
 
-// Index lists of routes 1 and 2 in index order, from the scanned block offsets.
+// The indices of routes 1 and 2 in index order, from the scanned block offsets: route
+// 1's first, then route 2's (from idx + totals[0]).
 __global__ __launch_bounds__(kGateBlock) void compact_routes(const int32_t* route, size_t n,
                                                              const uint32_t* blk_off,
-                                                             uint32_t* idx1, uint32_t* idx2) {
+                                                             const int* totals, uint32_t* idx) {
     __shared__ uint32_t wc[2][kGateBlock / 64];
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int r = i < n ? route[i] : 0;
@@ -670,7 +671,8 @@ __global__ __launch_bounds__(kGateBlock) void compact_routes(const int32_t* rout
         uint32_t base = blk_off[2 * blockIdx.x + k];
         for (int q = 0; q < w; ++q) base += wc[k][q];
         const uint64_t below = (1ull << lane) - 1;
-        (k == 0 ? idx1 : idx2)[base + __popcll((k == 0 ? v1 : v2) & below)] = (uint32_t)i;
+        const uint32_t at = (k == 0 ? 0u : (uint32_t)totals[0]) + base;
+        idx[at + __popcll((k == 0 ? v1 : v2) & below)] = (uint32_t)i;
     }
 }
 
@@ -1175,7 +1177,7 @@ int kwgpu_surrogate_price(kwgpu_ctx* ctx, kwgpu_surrogate* s, const kwgpu_option
         return q;
     };
     size_t need = 0;
-    for (size_t b : {sizeof(double), sizeof(double), (size_t)1, 2 * sizeof(uint32_t), sizeof(double),
+    for (size_t b : {sizeof(double), sizeof(double), (size_t)1, sizeof(uint32_t), sizeof(double),
                      sizeof(int32_t), sizeof(kwgpu_option), sizeof(int32_t), sizeof(int32_t),
                      sizeof(uint32_t)})
         need += (b * n + 255) / 256 * 256;
@@ -1194,7 +1196,7 @@ int kwgpu_surrogate_price(kwgpu_ctx* ctx, kwgpu_surrogate* s, const kwgpu_option
     double* prem = (double*)carve(base, at, sizeof(double));
     double* spr = (double*)carve(base, at, sizeof(double));
     uint8_t* inb = (uint8_t*)carve(base, at, 1);
-    uint32_t* idx = (uint32_t*)carve(base, at, 2 * sizeof(uint32_t));
+    uint32_t* idx = (uint32_t*)carve(base, at, sizeof(uint32_t));  // routes 1, then 2
     uint32_t* blk = (uint32_t*)carve(base, at, sizeof(uint32_t));  // 2 per 256 options
     double* sub_p = (double*)carve(base, at, sizeof(double));
     int32_t* sub_s = (int32_t*)carve(base, at, sizeof(int32_t));
@@ -1221,37 +1223,44 @@ int kwgpu_surrogate_price(kwgpu_ctx* ctx, kwgpu_surrogate* s, const kwgpu_option
                        serve_none ? nullptr : prem, spr, inb, spread_max, fp64_above, d_prices,
                        d_route, blk);
     hipLaunchKernelGGL(scan_block_counts, dim3(1), dim3(1024), 0, st, blk, nb, cnt);
-    hipLaunchKernelGGL(compact_routes, dim3(nb), dim3(kGateBlock), 0, st, d_route, n, blk, idx,
-                       idx + n);
+    hipLaunchKernelGGL(compact_routes, dim3(nb), dim3(kGateBlock), 0, st, d_route, n, blk,
+                       (const int*)cnt, idx);
     int hc[2] = {0, 0};
     if (hipMemcpyAsync(hc, cnt, sizeof(hc), hipMemcpyDeviceToHost, st) ||
         hipStreamSynchronize(st)) {
         cleanup();
         return kwgpu::ctx_fail(ctx, KWGPU_ERR_HIP, "kwgpu_surrogate_price: gate");
     }
-    for (int r = 0; r < 2 && rc == KWGPU_OK; ++r) {
-        const int m = hc[r];
-        if (m == 0) continue;
-        const uint32_t* ix = idx + (size_t)r * n;
-        hipLaunchKernelGGL(gather_opts, dim3(nblk(m, 256)), dim3(256), 0, st, d_opts, ix, m, sub_o);
+    // routes 1 and 2 in one FD solve (one uneven-batch plan): the compaction put route
+    // 2's options right after route 1's
+    const int m = hc[0] + hc[1];
+    if (m > 0) {
+        hipLaunchKernelGGL(gather_opts, dim3(nblk(m, 256)), dim3(256), 0, st, d_opts, idx, m, sub_o);
         kwgpu_fd1d_params p = *fd;
-        p.precision = r == 0 ? fd->precision : KWGPU_FP64;
+        p.precision = hc[0] ? fd->precision : KWGPU_FP64;
         // per-option grid sizes follow their options into the compacted subset
         if (fd->x_dims) {
-            hipLaunchKernelGGL(gather_dims, dim3(nblk(m, 256)), dim3(256), 0, st, fd->x_dims, ix, m,
-                               sub_x);
+            hipLaunchKernelGGL(gather_dims, dim3(nblk(m, 256)), dim3(256), 0, st, fd->x_dims, idx,
+                               m, sub_x);
             p.x_dims = sub_x;
         }
         if (fd->t_dims) {
-            hipLaunchKernelGGL(gather_dims, dim3(nblk(m, 256)), dim3(256), 0, st, fd->t_dims, ix, m,
-                               sub_t);
+            hipLaunchKernelGGL(gather_dims, dim3(nblk(m, 256)), dim3(256), 0, st, fd->t_dims, idx,
+                               m, sub_t);
             p.t_dims = sub_t;
         }
-        rc = kwgpu_fd1d_price_device(ctx, sub_o, (size_t)m, &p, sub_p, sub_s, st);
-        if (rc) break;
-        hipLaunchKernelGGL(scatter_prices, dim3(nblk(m, 256)), dim3(256), 0, st, sub_p, sub_s, ix, m,
-                           d_prices);
-        if (hipStreamSynchronize(st)) rc = kwgpu::ctx_fail(ctx, KWGPU_ERR_HIP, "FD fallback");
+        std::vector<uint8_t> prec;
+        if (hc[0] && hc[1] && fd->precision != KWGPU_FP64) {
+            prec.assign((size_t)m, (uint8_t)fd->precision);
+            std::fill(prec.begin() + hc[0], prec.end(), (uint8_t)KWGPU_FP64);
+        }
+        rc = kwgpu::fd_price_device_prec(ctx, sub_o, (size_t)m, &p, sub_p, sub_s, st,
+                                         prec.empty() ? nullptr : prec.data());
+        if (rc == KWGPU_OK) {
+            hipLaunchKernelGGL(scatter_prices, dim3(nblk(m, 256)), dim3(256), 0, st, sub_p, sub_s,
+                               idx, m, d_prices);
+            if (hipStreamSynchronize(st)) rc = kwgpu::ctx_fail(ctx, KWGPU_ERR_HIP, "FD fallback");
+        }
     }
     if (n_fd) *n_fd = (size_t)hc[0] + (size_t)hc[1];
     cleanup();

@@ -488,6 +488,30 @@ def test_mixed_validated_domain(ctx):
     assert np.all(hrel[over] <= EPS)  # the FP64 form itself
 
 
+def test_mixed_uneven_with_fp64_pdes(ctx):
+    """An uneven FP32-mixed batch through the host entry point with overflow-prone PDEs
+    (scale sigma sqrt(T) > 80) among the others: one plan holds buckets of both
+    precisions, those PDEs match the FP64 pricer, the rest stay within 1e-4, and a
+    repeated call (a graph replay) gives the same prices."""
+    _say("mixed uneven with FP64 PDEs")
+    o, xd, td = W.uneven(1500, 83)
+    td = np.minimum(td, 60)
+    risky = np.arange(len(o)) % 5 == 0
+    o["z"][risky] = 1.5
+    o["t"][risky] = 10.0
+    assert np.all(50.0 * o["z"][risky] * np.sqrt(o["t"][risky]) > 80.0)
+    p64, s64, rc, err = ctx.fd1d_price(o, x_dims=xd, t_dims=td)
+    assert rc == kwgpu.OK, err
+    p32, s32, rc, err = ctx.fd1d_price(o, x_dims=xd, t_dims=td, precision=kwgpu.FP32_MIXED)
+    assert rc == kwgpu.OK, err
+    assert np.all(s32 == kwgpu.STATUS_OK)
+    scale = np.maximum(1.0, np.abs(p64))
+    assert np.all(np.abs(p32 - p64)[risky] <= EPS * scale[risky])
+    assert np.all(np.abs(p32 - p64)[~risky] <= 1e-4 * scale[~risky])
+    again, _, rc, _ = ctx.fd1d_price(o, x_dims=xd, t_dims=td, precision=kwgpu.FP32_MIXED)
+    assert np.array_equal(again, p32)
+
+
 def test_mixed_full_size_per_class(ctx):
     """32768 options at 1024 x 1024: FP32-mixed against the FP64 GPU result (itself at
     parity with the reference), per class put/call x T < 1 / T >= 1."""

@@ -287,7 +287,7 @@ def bench_surrogate(ctx, args, log):
     serve = pipeline.serve(ctx, sur, opts, train["spread_bound"], log=log)
     serve["meets_bar"] = serve["max_abs_err_vs_fp64_fd"] <= pipeline.ERR_BAR
     # the same batch with per-option grids (x-points from {256, 512, 1024, 2048},
-    # t-points max(32, ceil(365 T))), judged by FP64 FD on those grids, with the FP64 and
+    # t-points max(32, ceil(365 T))), checked against FP64 FD on those grids, with the FP64 and
     # the FP32-mixed fallback
     xd, td = W.serving_grids(opts, 5)
     uneven = {}

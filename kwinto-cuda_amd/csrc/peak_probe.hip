@@ -1,5 +1,5 @@
 // kwgpu_peak -- measures, on the device it runs on, the two ceilings the FD pricer is
-// judged against: FP64 FMA throughput of the vector ALUs (independent FMA chains in
+// measured against: FP64 FMA throughput of the vector ALUs (independent FMA chains in
 // registers, no memory traffic) and device-to-device copy bandwidth (a streaming copy
 // kernel over 1 GiB). Prints one JSON line. Diagnostic only; not part of the library.
 #include <hip/hip_runtime.h>

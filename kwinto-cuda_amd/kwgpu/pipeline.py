@@ -1,5 +1,5 @@
 """Surrogate pipeline: labels from the GPU FD pricer, ensemble training, held-out
-evaluation judged by FD, gate calibration and surrogate-first serving.
+evaluation against FD, gate calibration and surrogate-first serving.
 
 Everything runs on the device through the kwgpu C-ABI; torch provides device memory
 and timing events only.
@@ -8,9 +8,9 @@ Workloads (kwgpu.workloads): the training set is `surrogate_train` (K U[40,200],
 T U[1/52,5], the plain-method sigma/r/q/w draws, seed 3); the serving batch is
 `serving` (the same box with ~10% of options moved outside it, seed 4). Labels use one
 grid, 1024 x-points x 1024 t-points. Serving uses that grid for the FD fallback too, so
-served prices and the FP64 FD prices they are judged against describe the same
+served prices and the FP64 FD prices they are compared with describe the same
 discretisation, or per-option grids (`serve(..., x_dims, t_dims)`), against which FP64
-FD on those same grids is the judge.
+FD on those same grids is the reference.
 """
 from __future__ import annotations
 
@@ -174,7 +174,7 @@ def serve(ctx: Context, sur: Surrogate, opts, spread_bound: float, repeats=3, lo
     """Surrogate-first pricing of `opts` against plain FP64 FD of the same batch:
     options/s of both, max/RMS error of the served prices vs FP64 FD. `x_dims`/`t_dims`
     (host int32 arrays, optional) give every option its own grid, for the FD fallback and
-    for the FP64 FD judge alike. `precision` is the fallback's; FP64 is the default since
+    for the FP64 FD reference alike. `precision` is the fallback's; FP64 is the default since
     on MI355X the FP64 form is both exact and the faster one."""
     torch = _torch()
     n = len(opts)
@@ -220,7 +220,7 @@ def serve(ctx: Context, sur: Surrogate, opts, spread_bound: float, repeats=3, lo
         ts.append(time.perf_counter() - t0)
     t_fd = min(ts)
     if int((d_st != 0).sum().item()):
-        raise RuntimeError("FP64 FD judge: failed options")
+        raise RuntimeError("FP64 FD reference: failed options")
     ref = d_ref.cpu().numpy()
     d = np.abs(served - ref)
     out = {

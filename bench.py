@@ -48,7 +48,7 @@ def parse_args():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--layout", default="wave", choices=["wave", "lane", "auto"])
-    ap.add_argument("--cpu-sample", type=int, default=1024,
+    ap.add_argument("--cpu-sample", type=int, default=8192,
                     help="options of the workload the CPU baseline prices (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--uneven", type=int, default=65536,
@@ -123,37 +123,61 @@ def cpu_model():
     return "unknown"
 
 
-def cpu_baseline(opts, sample, threads):
+def cpu_baseline(opts, sample, threads, gpu_prices):
     """The reference pricer itself (its sources compiled into oracle/_ref, its
     ThreadPool on every hardware thread) where that library was built, else the
-    oracle's restatement of it on `threads` threads; the other one is timed beside it."""
+    oracle's restatement of it on `threads` threads; the other one is timed beside it.
+
+    A leg's time is accepted only with its prices checked: the reference's
+    ThreadPool::wait can return before the last job has run
+    (src/kwThreadPool.cpp:59-72), which would show as unset prices and a faster CPU.
+    So the reference leg must equal the port bit for bit on every run, and both must
+    match the GPU FP64 prices of the same options to 1e-8."""
     import oracle
     sub = opts[:sample]
+    gpu = np.asarray(gpu_prices[:sample])
 
-    def timed(price):
+    def timed(price, reps):
         price(sub[:min(16, sample)])  # warm-up
-        runs = []
-        for _ in range(3):  # the median of three timed runs
+        runs, outs = [], []
+        for _ in range(reps):
             t0 = time.perf_counter()
-            _, err = price(sub)
+            p, err = price(sub)
             runs.append(time.perf_counter() - t0)
             if err:
                 raise RuntimeError(err)
-        return float(np.median(runs)), runs
+            outs.append(p.copy())
+        return float(np.median(runs)), runs, outs
 
     port = lambda s: oracle.fd1d_price(s, X_DIM, T_DIM, nthreads=threads)
-    legs = [("port", port, threads, "the oracle's restatement, one job per PDE")]
+    legs = [("port", port, threads, "the oracle's restatement, one job per PDE", 1)]
     if oracle.ref_available():
         ref = lambda s: oracle.ref_price("FD1D", s, X_DIM, T_DIM)
         legs.insert(0, ("reference", ref, oracle.ref_threads(),
-                        "the reference's Fd1d_Pricer and ThreadPool, compiled from its sources"))
-    out = None
-    for kind, price, cores, what in legs:
-        dt, runs = timed(price)
-        leg = {"value": sample / dt, "unit": "options/s", "cores": cores, "kind": kind,
-               "sample": f"first {sample} options of the same workload, {X_DIM}x{T_DIM} FP64, "
-                         f"{what}, {cores} threads, median of 3 runs "
-                         f"({', '.join(f'{r:.2f}' for r in runs)} s); host {cpu_model()}, "
+                        "the reference's Fd1d_Pricer and ThreadPool, compiled from its sources",
+                        3))
+    out, first_prices = None, None
+    for kind, price, cores, what, reps in legs:
+        dt, runs, outs = timed(price, reps)
+        for p in outs:
+            d_gpu = float(np.max(np.abs(p - gpu)))
+            if not d_gpu <= 1e-8:
+                raise RuntimeError(f"cpu_baseline {kind} leg: max |cpu - gpu| = {d_gpu:.3e}")
+            if first_prices is not None and not np.array_equal(p, first_prices):
+                raise RuntimeError(f"cpu_baseline {kind} leg: prices differ from the "
+                                   f"first leg's")
+        if first_prices is None:
+            first_prices = outs[0]
+        rate = sample / dt
+        leg = {"value": rate, "unit": "options/s", "cores": cores, "kind": kind,
+               "sample": f"first {sample} of the {len(opts)} options of the same workload "
+                         f"({100.0 * sample / len(opts):.0f} %), {X_DIM}x{T_DIM} FP64, "
+                         f"{what}, {cores} threads, "
+                         + (f"median of {reps} runs " if reps > 1 else "one run ")
+                         + f"({', '.join(f'{r:.2f}' for r in runs)} s); every run's prices "
+                         f"checked (equal to the other leg's bit for bit, within 1e-8 of the "
+                         f"GPU's); the whole batch at this rate: "
+                         f"{len(opts) / rate:.1f} s; host {cpu_model()}, "
                          f"{os.cpu_count()} logical CPUs visible"}
         if out is None:
             out = leg
@@ -210,7 +234,9 @@ def bench_uneven(ctx, torch, stream, n, steps, log):
         if not bool((d_s == 0).all()):
             raise RuntimeError("uneven batch: failed options")
         prices[name] = d_p.cpu().numpy().copy()
-        peak = FP64_PEAK_TFLOPS if prec == kwgpu.FP64 else 157.3
+        # FP32-mixed against the non-packed FP32 rate this build issues (packed FP32 is
+        # switched off), which equals the FP64 vector rate
+        peak = FP64_PEAK_TFLOPS
         out[name] = {"options_per_s": n / (ms * 1e-3), "ms_per_step": ms,
                      "tflops": flops / (ms * 1e-3) / 1e12,
                      "roofline_frac": flops / (ms * 1e-3) / 1e12 / peak,
@@ -420,7 +446,7 @@ def main():
         threads = min(args.cpu_threads, os.cpu_count() or 1)
         print(f"bench: CPU baseline, {args.cpu_sample} options on {threads} threads",
               file=sys.stderr, flush=True)
-        cpu = cpu_baseline(opts, args.cpu_sample, threads)
+        cpu = cpu_baseline(opts, args.cpu_sample, threads, p64)
         for f in forms.values():
             f["speedup_vs_cpu"] = f["options_per_s"] / cpu["value"]
 
@@ -447,7 +473,7 @@ def main():
             "forms": forms,
         },
         "roofline": {
-            "bound": "mfma",
+            "bound": "valu",
             "roof": "FP64 VALU (MI355X FP64 vector peak = FP64 dense peak = 78.6 TFLOP/s"
                     + ("; non-packed FP32 issues at the same rate)" if mixed_chosen else ")"),
             "achieved": achieved,

@@ -14,10 +14,13 @@
  *   kw::BlackScholes_Pricer::price           src/Pricer/kwBlackScholes.cpp:13-50
  *   kw::Fd1d_BlackScholes_Pricer::price      src/Pricer/kwFd1d_BlackScholes.cpp:15-43
  *
- * Threading: one context per host thread; every call on a context is
- * serialised on that context's HIP stream. Host-pointer entry points are
+ * Threading: one context per host thread. Host-pointer entry points are
  * synchronous (like the reference's blocking price()); *_device entry points
- * are asynchronous on the stream passed in (NULL = the context's stream).
+ * are asynchronous on the stream passed in (NULL = the context's stream). Calls
+ * on one context may use different streams: an FD solve first waits for the
+ * context's previous solve when that one ran on another stream (both reuse the
+ * context's device buffers), and a surrogate prediction likewise for the
+ * surrogate's previous one.
  */
 #ifndef KWGPU_H
 #define KWGPU_H
@@ -185,7 +188,8 @@ int kwgpu_fd1d_autotune(kwgpu_ctx* ctx, const kwgpu_option* d_opts, size_t n,
                         char* report, size_t report_len);
 
 /* Saves / loads the context's autotuned choices (text, one "bucket precision variant
- * persistent" line each). */
+ * persistent" line each). Loading skips lines of an unknown precision, variants that
+ * do not hold their bucket and lane-layout choices for FP32-mixed. */
 int kwgpu_tune_save(kwgpu_ctx* ctx, const char* path);
 int kwgpu_tune_load(kwgpu_ctx* ctx, const char* path);
 
@@ -252,7 +256,8 @@ int kwgpu_surrogate_predict(kwgpu_ctx* ctx, kwgpu_surrogate* s, const kwgpu_opti
  * sent to FP64). fd->x_dims / fd->t_dims, when set, are device arrays of n per-option
  * grid sizes, and each routed option is solved on its own. d_route[i] is 0 for the
  * surrogate, 1 for FD at fd->precision, 2 for FD in FP64 (with an FP64 fallback, every
- * option sent to FD). Options sent to FD are solved in index order. A negative
+ * option sent to FD). Options sent to FD are solved as one batch: route 1's in index
+ * order, then route 2's in index order. A negative
  * spread_max serves nothing and skips the ensemble.
  * *n_fd receives the number routed to FD. Device pointers;
  * synchronous (the FD subset is compacted on the device). Per-option work buffers are

@@ -67,6 +67,13 @@ struct kwgpu_ctx {
     std::vector<Bucket> plan;
     hipGraphExec_t graph = nullptr;
     bool use_graphs = true;
+    // the last launch of `graph` (drop_graph waits for it before destroying the graph)
+    hipEvent_t ev_graph = nullptr;
+    bool graph_in_flight = false;
+    // end of the last FD solve on this context and its stream: a solve on another stream
+    // waits for it before it reuses the context's PDE order, counters and workspace
+    hipEvent_t ev_done = nullptr;
+    hipStream_t done_stream = nullptr;
 };
 
 namespace {
@@ -241,9 +248,15 @@ int ensure_side_streams(kwgpu_ctx* c) {
     return KWGPU_OK;
 }
 
+// Destroys the cached graph once its last launch, which may still be queued on the
+// stream it was launched on, has finished.
 void drop_graph(kwgpu_ctx* c) {
-    if (c->graph) (void)hipGraphExecDestroy(c->graph);
+    if (c->graph) {
+        if (c->graph_in_flight) (void)hipEventSynchronize(c->ev_graph);
+        (void)hipGraphExecDestroy(c->graph);
+    }
     c->graph = nullptr;
+    c->graph_in_flight = false;
     c->plan_valid = false;
 }
 
@@ -257,8 +270,8 @@ void drop_graph(kwgpu_ctx* c) {
 // a persistent grid takes the longest first; each bucket's kernel (autotuned variant)
 // runs on its own side stream. The whole launch sequence is captured once into a
 // hipGraph and replayed while the batch (sizes, buffers, parameters) is unchanged.
-int run_fd(kwgpu_ctx* c, FdBatch b, const int32_t* pde_x, const int32_t* pde_t, int precision,
-           int layout, hipStream_t s, const uint8_t* pde_prec = nullptr) {
+int run_fd_on(kwgpu_ctx* c, FdBatch b, const int32_t* pde_x, const int32_t* pde_t, int precision,
+              int layout, hipStream_t s, const uint8_t* pde_prec) {
     // per-PDE precisions go through the planned path, with the uniform sizes spelled out
     std::vector<int32_t> ux, ut;
     if (pde_prec && !pde_x) {
@@ -419,7 +432,26 @@ int run_fd(kwgpu_ctx* c, FdBatch b, const int32_t* pde_x, const int32_t* pde_t, 
     }
     if (hipError_t e = hipGraphLaunch(c->graph, s); e != hipSuccess)
         return hip_fail(c, e, "kwgpu: graph launch");
+    if (hipError_t e = hipEventRecord(c->ev_graph, s); e != hipSuccess)
+        return hip_fail(c, e, "kwgpu: graph event");
+    c->graph_in_flight = true;
     return KWGPU_OK;
+}
+
+// run_fd_on, ordered after the context's previous solve: a solve on a stream other than
+// the previous one's first waits for that solve, whose kernels may still read the
+// context's PDE order, counters and lane workspace that this one rewrites; its own end
+// is recorded for the next.
+int run_fd(kwgpu_ctx* c, FdBatch b, const int32_t* pde_x, const int32_t* pde_t, int precision,
+           int layout, hipStream_t s, const uint8_t* pde_prec = nullptr) {
+    if (c->done_stream && c->done_stream != s)
+        if (hipError_t e = hipStreamWaitEvent(s, c->ev_done, 0); e != hipSuccess)
+            return hip_fail(c, e, "kwgpu: order after the previous solve");
+    const int rc = run_fd_on(c, b, pde_x, pde_t, precision, layout, s, pde_prec);
+    if (hipError_t e = hipEventRecord(c->ev_done, s); e != hipSuccess)
+        return rc ? rc : hip_fail(c, e, "kwgpu: solve event");
+    c->done_stream = s;
+    return rc;
 }
 
 // PDE compression (src/Pricer/kwFd1d.cpp:28-65): options equal in (t, r, q, z, e, w) and
@@ -706,6 +738,13 @@ int kwgpu_create(kwgpu_ctx** out, int device) {
         delete c;
         return KWGPU_ERR_HIP;
     }
+    if (hipEventCreateWithFlags(&c->ev_graph, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_done, hipEventDisableTiming) != hipSuccess) {
+        if (c->ev_graph) (void)hipEventDestroy(c->ev_graph);
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return KWGPU_ERR_HIP;
+    }
     *out = c;
     return KWGPU_OK;
 }
@@ -714,12 +753,15 @@ void kwgpu_destroy(kwgpu_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
+    if (c->done_stream) (void)hipEventSynchronize(c->ev_done);  // a solve on a caller's stream
     drop_graph(c);
     for (int k = 0; k < kSide; ++k) {
         if (c->side[k]) (void)hipStreamDestroy(c->side[k]);
         if (c->ev_join[k]) (void)hipEventDestroy(c->ev_join[k]);
     }
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    (void)hipEventDestroy(c->ev_graph);
+    (void)hipEventDestroy(c->ev_done);
     if (c->hsizes) (void)hipHostFree(c->hsizes);
     for (DevBuf* b : {&c->opts, &c->prices, &c->status, &c->rep, &c->off, &c->list, &c->order,
                       &c->xd, &c->td, &c->ws, &c->counters})
@@ -1162,11 +1204,15 @@ int kwgpu_tune_load(kwgpu_ctx* c, const char* path) {
     if (!f) return fail(c, KWGPU_ERR_INVALID, std::string("cannot read ") + path);
     int cap, prec, v, per;
     while (std::fscanf(f, "%d %d %d %d", &cap, &prec, &v, &per) == 4) {
+        // lines of another precision, and lane-layout choices for FP32-mixed (which runs
+        // in the wave layout only), are skipped: they could only make later calls fail
+        if (prec != KWGPU_FP64 && prec != KWGPU_FP32_MIXED) continue;
         if (cap == -1) {
             c->serial[prec] = v != 0;
             continue;
         }
         if (v >= kwgpu::wave_variant_count() || (v >= 0 && kwgpu::wave_variant(v).capacity() != cap)) continue;
+        if (v < 0 && prec != KWGPU_FP64) continue;
         c->tuned[{cap, prec}] = Choice{v, per != 0};
     }
     std::fclose(f);

@@ -13,6 +13,9 @@
 #include "kwgpu.h"
 
 extern "C" int mock_live();
+// the stand-in runtime's streams (hipStream_t is a pointer; hipError_t an int enum)
+extern "C" int hipStreamCreateWithFlags(void** s, unsigned flags);
+extern "C" int hipStreamDestroy(void* s);
 
 namespace {
 
@@ -63,10 +66,11 @@ int host_price(kwgpu_ctx* c, const std::vector<kwgpu_option>& o, kwgpu_fd1d_para
     return kwgpu_fd1d_price(c, o.data(), o.size(), &p, pr.data(), st.data());
 }
 
-int device_price(kwgpu_ctx* c, const std::vector<kwgpu_option>& o, const kwgpu_fd1d_params& p) {
+int device_price(kwgpu_ctx* c, const std::vector<kwgpu_option>& o, const kwgpu_fd1d_params& p,
+                 void* stream = nullptr) {
     std::vector<double> pr(o.size());
     std::vector<int32_t> st(o.size());
-    return kwgpu_fd1d_price_device(c, o.data(), o.size(), &p, pr.data(), st.data(), nullptr);
+    return kwgpu_fd1d_price_device(c, o.data(), o.size(), &p, pr.data(), st.data(), stream);
 }
 
 }  // namespace
@@ -166,7 +170,34 @@ int main(int argc, char** argv) {
         expect(kwgpu_create(&c2, 0), KWGPU_OK, c2, "second context");
         expect(kwgpu_tune_load(c2, path.c_str()), KWGPU_OK, c2, "tune load");
         expect(device_price(c2, o, p), KWGPU_OK, c2, "loaded choices");
+        // a replay on the context's stream, then the same batch on two other streams: each
+        // solve waits for the previous one (same graph, same PDE order and counters)
+        void* sa = nullptr;
+        void* sb = nullptr;
+        hipStreamCreateWithFlags(&sa, 1);
+        hipStreamCreateWithFlags(&sb, 1);
+        std::vector<double> rp(o.size());
+        std::vector<int32_t> rs(o.size());
+        for (void* st : {(void*)nullptr, sa, sb, sa})
+            expect(kwgpu_fd1d_price_device(c2, o.data(), o.size(), &p, rp.data(), rs.data(), st),
+                   KWGPU_OK, c2, "replay on a stream");
+        // loading choices drops the graph whose launch may still be queued
+        expect(kwgpu_tune_load(c2, path.c_str()), KWGPU_OK, c2, "tune load after a launch");
+        // a file with a foreign precision and a lane-layout choice for FP32-mixed: both
+        // lines are skipped, and FP32-mixed batches still price
+        const std::string bad = dir + "/tune_bad.txt";
+        if (FILE* f = std::fopen(bad.c_str(), "w")) {
+            std::fprintf(f, "256 7 2 0\n512 %d -1 0\n1024 %d -1 1\n-1 9 1 0\n",
+                         KWGPU_FP32_MIXED, KWGPU_FP32_MIXED);
+            std::fclose(f);
+        }
+        expect(kwgpu_tune_load(c2, bad.c_str()), KWGPU_OK, c2, "tune load, bad lines");
+        kwgpu_fd1d_params pm = p;
+        pm.precision = KWGPU_FP32_MIXED;
+        expect(device_price(c2, o, pm, sb), KWGPU_OK, c2, "mixed after a bad tune file");
         kwgpu_destroy(c2);
+        hipStreamDestroy(sa);
+        hipStreamDestroy(sb);
     }
     {  // PDE grouping: ten (spot, strike) pairs per PDE
         auto base = options(60, 7);

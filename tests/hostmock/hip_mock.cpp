@@ -4,16 +4,37 @@
 // graph executables are heap tokens that abort on use after destroy, double destroy or
 // an unknown handle. Built with AddressSanitizer, this turns host-side memory errors of
 // the library (its uneven-batch plans, grouping, staging, serving buffers) into test
-// failures. Test infrastructure only.
+// failures.
+//
+// Work queued on a stream is modelled as still running until something the host waits
+// on covers it: hipStreamSynchronize of that stream (or of a stream that waited on an
+// event recorded after it), hipEventSynchronize of an event recorded after it, or
+// hipDeviceSynchronize. Destroying a graph executable whose launch is still running
+// aborts, as does a hipStreamWaitEvent on an event never recorded, and so does a launch
+// of a graph executable on one stream while its launch on another may still run and the
+// first stream's work was not waited for. Test infrastructure only.
 #include <hip/hip_runtime.h>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <set>
 
 static std::set<void*> g_dev, g_host, g_graph, g_exec, g_stream, g_event;
 static bool g_capturing = false;
 static long g_launches = 0, g_graph_launches = 0;
+// graph executables whose launches may still run: per stream, and as captured by each
+// recorded event
+static std::map<void*, std::set<void*>> g_pending;
+static std::map<void*, std::set<void*>> g_event_work;
+static std::set<void*> g_recorded;
+
+static void complete(const std::set<void*>& done) {
+    for (auto& [s, work] : g_pending)
+        for (void* x : done) work.erase(x);
+    for (auto& [e, work] : g_event_work)
+        for (void* x : done) work.erase(x);
+}
 
 static void* token(std::set<void*>& s) {
     void* p = std::malloc(16);
@@ -60,7 +81,11 @@ hipError_t hipLaunchKernel(const void*, dim3 g, dim3 b, void**, size_t shm, hipS
     ++g_launches;
     return hipSuccess;
 }
-hipError_t hipDeviceSynchronize() { return hipSuccess; }
+hipError_t hipDeviceSynchronize() {
+    g_pending.clear();
+    g_event_work.clear();
+    return hipSuccess;
+}
 hipError_t hipGetLastError() { return hipSuccess; }
 const char* hipGetErrorString(hipError_t) { return "mock error"; }
 hipError_t hipGetDeviceCount(int* n) {
@@ -124,20 +149,29 @@ hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned int) {
 }
 hipError_t hipStreamDestroy(hipStream_t s) {
     untoken(g_stream, s, "hipStreamDestroy");
+    g_pending.erase(s);
     return hipSuccess;
 }
-hipError_t hipStreamSynchronize(hipStream_t) {
+hipError_t hipStreamSynchronize(hipStream_t s) {
     if (g_capturing) {
         std::fprintf(stderr, "MOCK: sync during capture\n");
         std::abort();
     }
+    const std::set<void*> done = g_pending[s];
+    complete(done);
     return hipSuccess;
 }
-hipError_t hipStreamWaitEvent(hipStream_t, hipEvent_t e, unsigned int) {
+hipError_t hipStreamWaitEvent(hipStream_t s, hipEvent_t e, unsigned int) {
     if (e && !g_event.count(e)) {
         std::fprintf(stderr, "MOCK: wait on unknown event\n");
         std::abort();
     }
+    if (e && !g_recorded.count(e)) {
+        std::fprintf(stderr, "MOCK: wait on an event never recorded\n");
+        std::abort();
+    }
+    // s's later work runs after the event's, so waiting for s covers it too
+    if (e) g_pending[s].insert(g_event_work[e].begin(), g_event_work[e].end());
     return hipSuccess;
 }
 hipError_t hipEventCreate(hipEvent_t* e) {
@@ -147,16 +181,24 @@ hipError_t hipEventCreate(hipEvent_t* e) {
 hipError_t hipEventCreateWithFlags(hipEvent_t* e, unsigned) { return hipEventCreate(e); }
 hipError_t hipEventDestroy(hipEvent_t e) {
     untoken(g_event, e, "hipEventDestroy");
+    g_event_work.erase(e);
+    g_recorded.erase(e);
     return hipSuccess;
 }
-hipError_t hipEventRecord(hipEvent_t e, hipStream_t) {
+hipError_t hipEventRecord(hipEvent_t e, hipStream_t s) {
     if (!g_event.count(e)) {
         std::fprintf(stderr, "MOCK: record of unknown event\n");
         std::abort();
     }
+    g_recorded.insert(e);
+    g_event_work[e] = g_pending[s];
     return hipSuccess;
 }
-hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }
+hipError_t hipEventSynchronize(hipEvent_t e) {
+    const std::set<void*> done = g_event_work[e];
+    complete(done);
+    return hipSuccess;
+}
 hipError_t hipEventElapsedTime(float* ms, hipEvent_t, hipEvent_t) {
     *ms = 1.0f;
     return hipSuccess;
@@ -187,15 +229,29 @@ hipError_t hipGraphInstantiate(hipGraphExec_t* x, hipGraph_t g, hipGraphNode_t*,
     return hipSuccess;
 }
 hipError_t hipGraphExecDestroy(hipGraphExec_t x) {
+    for (const auto& [s, work] : g_pending)
+        if (work.count(x)) {
+            std::fprintf(stderr, "MOCK: hipGraphExecDestroy of %p while a launch of it may "
+                                 "still run on stream %p\n", (void*)x, s);
+            std::abort();
+        }
     untoken(g_exec, x, "hipGraphExecDestroy");
     return hipSuccess;
 }
-hipError_t hipGraphLaunch(hipGraphExec_t x, hipStream_t) {
+hipError_t hipGraphLaunch(hipGraphExec_t x, hipStream_t s) {
     if (!g_exec.count(x)) {
         std::fprintf(stderr, "MOCK: launch of unknown graph exec %p\n", x);
         std::abort();
     }
+    for (const auto& [other, work] : g_pending)
+        if (other != (void*)s && work.count(x) && !g_pending[s].count(x)) {
+            std::fprintf(stderr, "MOCK: graph exec %p launched on stream %p while its launch on "
+                                 "stream %p may still run, with no wait between them\n",
+                         (void*)x, (void*)s, other);
+            std::abort();
+        }
     ++g_graph_launches;
+    g_pending[s].insert(x);
     return hipSuccess;
 }
 }

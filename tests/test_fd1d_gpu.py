@@ -435,14 +435,15 @@ def test_mixed_sizes_vs_oracle(ctx, x_dim, t_dim):
     p, st, rc, err = ctx.fd1d_price(o, x_dim, t_dim, precision=kwgpu.FP32_MIXED)
     assert np.array_equal(st, rst)
     ok = rst == 0
-    assert np.all(np.abs(p[ok] - ref[ok]) <= MIXED_TOL * np.maximum(1.0, np.abs(ref[ok])))
+    assert np.all(np.abs(p[ok] - ref[ok]) <= MIXED_TOL)  # absolute, as the bar is stated
 
 
 def test_mixed_validated_domain(ctx):
     """FP32-mixed over a wide random box at 1024 x 1024 (vol 2 %-120 %, maturity
     0.002-15 years, rates -3 %-20 %, yields 0-25 %, spot 0.4-2.5x strike). Device entry
-    point (one solve per option, as asked): within 1e-4 relative of FP64 wherever
-    scale sigma sqrt(T) < 80 (measured: at most 3.2e-8); above that, calls whose FP32
+    point (one solve per option, as asked): within 1e-4 *relative* to max(1, |FP64
+    price|) wherever scale sigma sqrt(T) < 80 (prices here reach the hundreds; measured:
+    at most 3.2e-8 relative); above that, calls whose FP32
     increments overflow are flagged non-finite, and no price is silently outside the bar.
     Host entry point: those PDEs are solved in FP64, so every option is priced."""
     import torch
@@ -507,7 +508,7 @@ def test_mixed_uneven_with_fp64_pdes(ctx):
     assert np.all(s32 == kwgpu.STATUS_OK)
     scale = np.maximum(1.0, np.abs(p64))
     assert np.all(np.abs(p32 - p64)[risky] <= EPS * scale[risky])
-    assert np.all(np.abs(p32 - p64)[~risky] <= 1e-4 * scale[~risky])
+    assert np.abs(p32 - p64)[~risky].max() <= MIXED_TOL  # absolute
     again, _, rc, _ = ctx.fd1d_price(o, x_dims=xd, t_dims=td, precision=kwgpu.FP32_MIXED)
     assert np.array_equal(again, p32)
 

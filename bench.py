@@ -58,9 +58,13 @@ def parse_args():
                     help="also train the premium ensemble on FD labels and time "
                          "surrogate-first serving (0 = skip)")
     ap.add_argument("--sur-train", type=int, default=2_000_000,
-                    help="labelled options (FD FP64 1024x1024); 5 %% of them are held out")
+                    help="training options, each labelled at its two bracketing mesh nodes "
+                         "(FD FP64 1024x1024); 5 %% of them are held out")
     ap.add_argument("--sur-serve", type=int, default=1 << 20)
-    ap.add_argument("--sur-epochs", type=int, default=40)
+    ap.add_argument("--sur-epochs", type=int, default=160)
+    ap.add_argument("--sur-per-option", type=int, default=1,
+                    help="also fit a surrogate to the per-option-grid FD pricer and serve "
+                         "the batch on per-option grids")
     ap.add_argument("--next-rows", type=int, default=1,
                     help="also time FD1D-BS, a compressed chain book and Greeks (host entry "
                          "points, copies included)")
@@ -303,6 +307,12 @@ def bench_next_rows(ctx, opts, cpu_threads, log):
 
 
 def bench_surrogate(ctx, args, log):
+    """Surrogate training (2M options, FD FP64 labels at 1024 x 1024, 4 members) and
+    surrogate-first serving of the 1M-option batch: on the 1024 x 1024 grid the surrogate
+    was fitted to, with the FP64 and the FP32-mixed FD fallback; on per-option grids
+    (x-points from {256, 512, 1024, 2048}, t-points max(32, ceil(365 T))) with a second
+    ensemble fitted to the FD pricer on such grids. Each is checked against FP64 FD on the
+    same grids."""
     import kwgpu
     from kwgpu import pipeline
     from kwgpu import workloads as W
@@ -310,18 +320,11 @@ def bench_surrogate(ctx, args, log):
     sur, train = pipeline.train_and_evaluate(ctx, n_train=args.sur_train - n_test, n_test=n_test,
                                              epochs=args.sur_epochs, log=log)
     opts = W.serving(args.sur_serve, 4)
-    serve = pipeline.serve(ctx, sur, opts, train["spread_bound"], log=log)
+    bound = train["spread_bound"]
+    serve = pipeline.serve(ctx, sur, opts, bound, log=log)
     serve["meets_bar"] = serve["max_abs_err_vs_fp64_fd"] <= pipeline.ERR_BAR
-    # the same batch with per-option grids (x-points from {256, 512, 1024, 2048},
-    # t-points max(32, ceil(365 T))), checked against FP64 FD on those grids, with the FP64 and
-    # the FP32-mixed fallback
-    xd, td = W.serving_grids(opts, 5)
-    uneven = {}
-    for name, prec in (("fp64", kwgpu.FP64), ("fp32_mixed", kwgpu.FP32_MIXED)):
-        r = pipeline.serve(ctx, sur, opts, train["spread_bound"], log=log, x_dims=xd,
-                           t_dims=td, precision=prec)
-        r["meets_bar"] = r["max_abs_err_vs_fp64_fd"] <= pipeline.ERR_BAR
-        uneven[name + "_fallback"] = r
+    serve32 = pipeline.serve(ctx, sur, opts, bound, log=log, precision=kwgpu.FP32_MIXED)
+    serve32["meets_bar"] = serve32["max_abs_err_vs_fp64_fd"] <= pipeline.ERR_BAR
     # the trade-off at looser bars (not the metric: its bar is 1e-4)
     looser = {}
     for bar, v in train["looser_bars"].items():
@@ -329,8 +332,21 @@ def bench_surrogate(ctx, args, log):
         r["meets_bar"] = r["max_abs_err_vs_fp64_fd"] <= float(bar)
         looser[bar] = r
     sur.close()
-    return {"training": train, "serving": serve, "serving_per_option_grids": uneven,
-            "serving_at_looser_bars": looser}
+    uneven = {}
+    if args.sur_per_option:
+        sur2, train2 = pipeline.train_and_evaluate(ctx, n_train=args.sur_train - n_test,
+                                                   n_test=n_test, epochs=args.sur_epochs,
+                                                   per_option_grids=True, log=log)
+        xd, td = W.serving_grids(opts, 5)
+        uneven["training"] = train2
+        for name, prec in (("fp64", kwgpu.FP64), ("fp32_mixed", kwgpu.FP32_MIXED)):
+            r = pipeline.serve(ctx, sur2, opts, train2["spread_bound"], log=log, x_dims=xd,
+                               t_dims=td, precision=prec)
+            r["meets_bar"] = r["max_abs_err_vs_fp64_fd"] <= pipeline.ERR_BAR
+            uneven[name + "_fallback"] = r
+        sur2.close()
+    return {"training": train, "serving": serve, "serving_fp32_mixed_fallback": serve32,
+            "serving_per_option_grids": uneven, "serving_at_looser_bars": looser}
 
 
 def main():
@@ -433,10 +449,14 @@ def main():
             log)
     if args.surrogate:
         extra["surrogate"] = bench_surrogate(ctx, args, log)
+        sv = extra["surrogate"]["serving"]
         forms["surrogate_first"] = {
-            "options_per_s": extra["surrogate"]["serving"]["surrogate_first_options_per_s"],
-            "max_abs_err_vs_fp64_fd": extra["surrogate"]["serving"]["max_abs_err_vs_fp64_fd"],
-            "workload": f"{args.sur_serve} serving options (seed 4), 1024x1024 FD fallback"}
+            "options_per_s": sv["surrogate_first_options_per_s"],
+            "max_abs_err_vs_fp64_fd": sv["max_abs_err_vs_fp64_fd"],
+            "served_by_surrogate": sv["served_by_surrogate"],
+            "speedup_vs_fd_fp64_same_batch": sv["speedup_vs_fd_fp64"],
+            "workload": f"{args.sur_serve} serving options (seed 4), 1024x1024, FP64 FD "
+                        f"fallback"}
 
     if args.peak_probe:
         extra["device_ceilings"] = device_ceilings(achieved)

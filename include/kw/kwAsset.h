@@ -3,6 +3,8 @@
 #pragma once
 
 #include <cstddef>
+#include <cstdio>
+#include <utility>
 #include <ostream>
 #include <string>
 #include <tuple>
@@ -12,33 +14,32 @@
 
 namespace kw {
 
+// Field order, widths and names are the reference's (its callers and the 56-byte layout
+// depend on them). Units: t in years, k and s in currency, z, r and q as annual
+// fractions (0.2 is 20 %); e is 1 for early exercise, w is -1 for a put and +1 for a call.
 struct Option {
-    f64 t;  /// time to maturity     1.0 = 365 days
-    f64 k;  /// strike               100 = $100
-    f64 z;  /// volatility           0.2 = 20% per year
-    f64 r;  /// interest rate        0.05 = 5% per year
-    f64 q;  /// dividend rate        0.03 = 3% per year
-    f64 s;  /// spot price           100 = $100
-    u8 e;   /// early exercise       0 = European, 1 = American
-    i8 w;   /// parity               -1 = put, +1 call
+    f64 t;
+    f64 k;
+    f64 z;
+    f64 r;
+    f64 q;
+    f64 s;
+    u8 e;
+    i8 w;
 
+    // ordering used to sort portfolios: (t, k, z, q, r, s, e, w) lexicographically
     friend bool operator<(const Option& l, const Option& r) {
         return std::tie(l.t, l.k, l.z, l.q, l.r, l.s, l.e, l.w) <
                std::tie(r.t, r.k, r.z, r.q, r.r, r.s, r.e, r.w);
     }
 
+    // One-line description in the reference's format: exercise, then k q r s t with
+    // six decimals (std::to_string's "%f"), then the parity, then z.
     std::string asString() const {
-        std::string res = "<Option";
-        res += e ? " e=amer" : " e=euro";
-        res += " k=" + std::to_string(k);
-        res += " q=" + std::to_string(q);
-        res += " r=" + std::to_string(r);
-        res += " s=" + std::to_string(s);
-        res += " t=" + std::to_string(t);
-        res += w > 0 ? " w=Call" : " w=Put";
-        res += " z=" + std::to_string(z);
-        res += ">";
-        return res;
+        char buf[256];
+        std::snprintf(buf, sizeof(buf), "<Option e=%s k=%f q=%f r=%f s=%f t=%f w=%s z=%f>",
+                      e ? "amer" : "euro", k, q, r, s, t, w > 0 ? "Call" : "Put", z);
+        return buf;
     }
 };
 
@@ -48,10 +49,13 @@ static_assert(offsetof(Option, e) == offsetof(kwgpu_option, e) &&
                   offsetof(Option, s) == offsetof(kwgpu_option, s),
               "kw::Option must match kwgpu_option");
 
+// Stream form (the reference's format: spot and expiry first, comma-separated, stream
+// precision for the numbers).
 inline std::ostream& operator<<(std::ostream& os, const Option& o) {
-    return os << "<Option s=" << o.s << " t=" << o.t << ", k=" << o.k << ", z=" << o.z
-              << ", r=" << o.r << ", q=" << o.q << ", " << (o.e ? "amer" : "euro") << ", "
-              << (o.w > 0 ? "call" : "put") << ">";
+    os << "<Option s=" << o.s << " t=" << o.t;
+    const std::pair<const char*, f64> rest[] = {{"k", o.k}, {"z", o.z}, {"r", o.r}, {"q", o.q}};
+    for (const auto& [name, v] : rest) os << ", " << name << "=" << v;
+    return os << ", " << (o.e ? "amer" : "euro") << ", " << (o.w > 0 ? "call" : "put") << ">";
 }
 
 }  // namespace kw

@@ -203,22 +203,28 @@ void* kwgpu_stream(kwgpu_ctx* ctx);
  * Learned early-exercise premium (no counterpart in the reference; it sits in front
  * of the FD pricer above and uses it as its source of labels and as its fallback).
  *
- * An ensemble of MLPs 6 -> 256 -> 256 -> 256 -> 256 -> 1 with SiLU predicts
- * y = (premium - baseline) / (baseline + 1e-4 K), where premium = FD American price -
- * Black-Scholes European price and baseline is the Barone-Adesi & Whaley quadratic
- * approximation of that premium (kwgpu_surrogate_baseline; a control variate that leaves
- * the network a fraction of the premium to learn, and whose size sets the price scale of
- * the network's error), from features (log(S/K), T, sigma, r, q, w) normalised to the box
- * K/S in [0.4, 2], T in [1/52, 5], sigma in [0.1, 0.6], r in [0, 0.1], q in [0, 0.08].
+ * A surrogate emulates one FD pricer configuration: density, scale, theta and either
+ * one x_dim x t_dim grid or per-option grids. An ensemble of MLPs 8 -> 256 -> 256 ->
+ * 256 -> 256 -> 1 with SiLU predicts, at a mesh node x_j of an option's own grid, the
+ * FD value there minus the Black-Scholes price, as y = (premium - baseline) /
+ * (baseline + 1e-4 K), where baseline is the Barone-Adesi & Whaley quadratic
+ * approximation of the premium (kwgpu_surrogate_baseline; a control variate whose size
+ * sets the price scale of the network's error). Features: log(S/K), T, sigma, r, q, w
+ * normalised to the box K/S in [0.4, 2], T in [1/52, 5], sigma in [0.1, 0.6],
+ * r in [0, 0.1], q in [0, 0.08], and log2 of the grid's x and t points.
+ * A price is formed as the FD pricer forms its own (Fd1d::value,
+ * src/Math/kwFd1d.cpp:139-158): the network at the two nodes bracketing log(S/K), BS
+ * added at each, linear interpolation with the reference's weights. The labels are the
+ * FD pricer's values at those nodes.
  * The three 256x256 layers run on BF16 MFMA with FP32 accumulation (forward and
- * backward); the 6-wide input layer and the 1-wide output layer run in FP32; Adam
+ * backward); the 8-wide input layer and the 1-wide output layer run in FP32; Adam
  * keeps FP32 master weights.
  * --------------------------------------------------------------------------------- */
 typedef struct kwgpu_surrogate kwgpu_surrogate;
 
-/* Parameters per member, in this order: W1[256][6], b1[256], W2[256][256], b2[256],
+/* Parameters per member, in this order: W1[256][8], b1[256], W2[256][256], b2[256],
  * W3[256][256], b3[256], W4[256][256], b4[256], W5[256], b5[1]. */
-#define KWGPU_SURROGATE_PARAMS 199425
+#define KWGPU_SURROGATE_PARAMS 199937
 
 typedef struct kwgpu_surrogate_train_params {
     int32_t epochs;     /* passes over the training set                         */
@@ -231,34 +237,66 @@ typedef struct kwgpu_surrogate_train_params {
 
 void kwgpu_surrogate_train_params_default(kwgpu_surrogate_train_params* p);
 
-/* Creates `members` networks with seeded (He-uniform) initial weights. */
+/* Creates `members` networks with seeded (He-uniform) initial weights. The surrogate
+ * emulates the reference defaults on a 1024 x 1024 grid until kwgpu_surrogate_fit
+ * sets the configuration. */
 int kwgpu_surrogate_create(kwgpu_ctx* ctx, int32_t members, uint64_t seed,
                            kwgpu_surrogate** out);
 void kwgpu_surrogate_destroy(kwgpu_surrogate* s);
 
-/* Trains every member on (options, FD premium) pairs held on the device; member m
- * shuffles with seed + m. loss_out (host, may be NULL) receives epochs x members
- * mean training losses. Synchronous. */
+/* Labels for fitting a surrogate to the FD pricer configured by `fd` (precision FP64;
+ * fd->x_dims / fd->t_dims, when set, device arrays of n per-option grid sizes): option i
+ * gives two node options, d_nodes[i] and d_nodes[n + i], equal to it but with spot
+ * K e^{x_{j-1}} and K e^{x_j} at the mesh nodes that bracket its log(S/K) on its own grid
+ * (the first node x_j >= log(S/K), as Fd1d::value brackets, src/Math/kwFd1d.cpp:139-158);
+ * d_premium[r] is the FD price of node option r (the FD value at that node) minus its
+ * Black-Scholes European price. d_keep[i] is 0 where option i has no such bracket (spot
+ * outside the grid or in its first cell) or a node failed to price. Device pointers
+ * (d_nodes 2n, d_premium 2n, d_keep n); synchronous. */
+int kwgpu_surrogate_labels(kwgpu_ctx* ctx, const kwgpu_option* d_opts, size_t n,
+                           const kwgpu_fd1d_params* fd, kwgpu_option* d_nodes, double* d_premium,
+                           uint8_t* d_keep);
+
+/* Fits the surrogate to the FD pricer configured by `fd` (precision FP64; fd->x_dims /
+ * fd->t_dims, when set, device arrays of n per-option grid sizes): each option's two
+ * node options (kwgpu_surrogate_labels) become two labelled rows; options with
+ * d_keep 0 are dropped. Then trains every member on the 2 * (*n_pairs) rows;
+ * member m shuffles with seed + m. loss_out (host, may be NULL) receives epochs x
+ * members mean training losses; *n_pairs (may be NULL) the options kept. The surrogate
+ * records `fd` as the configuration it emulates. Synchronous. */
+int kwgpu_surrogate_fit(kwgpu_ctx* ctx, kwgpu_surrogate* s, const kwgpu_option* d_opts, size_t n,
+                        const kwgpu_fd1d_params* fd, const kwgpu_surrogate_train_params* p,
+                        float* loss_out, size_t* n_pairs);
+
+/* Trains every member on (option, premium) rows held on the device, each row's features
+ * taken on the surrogate's single grid (not for a per-option surrogate); loss_out as for
+ * kwgpu_surrogate_fit. Synchronous. */
 int kwgpu_surrogate_train(kwgpu_ctx* ctx, kwgpu_surrogate* s, const kwgpu_option* d_opts,
                           const double* d_premium, size_t n,
                           const kwgpu_surrogate_train_params* p, float* loss_out);
 
-/* Ensemble inference: premium (price units, mean over members), spread (max - min over
- * members, price units) and whether the option lies in the training box. Device
- * pointers, asynchronous on `stream` (NULL = the context's stream). */
+/* Ensemble inference at each option's spot, formed from the two bracketing nodes as
+ * above: premium (price units: the mean surrogate price minus BS at the spot), spread
+ * (max - min of the members' prices) and whether the option lies in the training box
+ * with a labelled bracket. `fd` (may be NULL for a single-grid surrogate) must be the
+ * configuration the surrogate emulates and gives per-option grids. Device pointers,
+ * asynchronous on `stream` (NULL = the context's stream); a prediction on another
+ * stream than the surrogate's previous one waits for that one. */
 int kwgpu_surrogate_predict(kwgpu_ctx* ctx, kwgpu_surrogate* s, const kwgpu_option* d_opts,
-                            size_t n, double* d_premium, double* d_spread, uint8_t* d_in_box,
-                            void* stream);
+                            size_t n, const kwgpu_fd1d_params* fd, double* d_premium,
+                            double* d_spread, uint8_t* d_in_box, void* stream);
 
 /* Surrogate-first pricing: options in the box whose ensemble spread is <= spread_max
- * are priced as BS + premium; the rest go to the FD pricer with `fd` (fd->precision
- * FP64, or FP32_MIXED with options whose exp(scale sigma sqrt(T)) would overflow FP32
- * sent to FP64). fd->x_dims / fd->t_dims, when set, are device arrays of n per-option
- * grid sizes, and each routed option is solved on its own. d_route[i] is 0 for the
- * surrogate, 1 for FD at fd->precision, 2 for FD in FP64 (with an FP64 fallback, every
- * option sent to FD). Options sent to FD are solved as one batch: route 1's in index
- * order, then route 2's in index order. A negative
- * spread_max serves nothing and skips the ensemble.
+ * are priced by the surrogate (BS + premium); the rest go to the FD pricer with `fd`
+ * (fd->precision FP64, or FP32_MIXED with options whose exp(scale sigma sqrt(T)) would
+ * overflow FP32 sent to FP64). fd->x_dims / fd->t_dims, when set, are device arrays of
+ * n per-option grid sizes, and each routed option is solved on its own. The surrogate
+ * serves only when `fd` is the configuration it emulates (same density, scale, theta
+ * and grid, or per-option grids for a per-option surrogate); otherwise, or with a
+ * negative spread_max, every option goes to FD and the ensemble is skipped.
+ * d_route[i] is 0 for the surrogate, 1 for FD at fd->precision, 2 for FD in FP64 (with
+ * an FP64 fallback, every option sent to FD). Options sent to FD are solved as one
+ * batch: route 1's in index order, then route 2's in index order.
  * *n_fd receives the number routed to FD. Device pointers;
  * synchronous (the FD subset is compacted on the device). Per-option work buffers are
  * kept with `s` and grown on demand. */
@@ -272,7 +310,7 @@ int kwgpu_surrogate_get_params(kwgpu_ctx* ctx, const kwgpu_surrogate* s, int32_t
 int kwgpu_surrogate_set_params(kwgpu_ctx* ctx, kwgpu_surrogate* s, int32_t member,
                                const float* host_params);
 
-/* One forward/backward pass of `member` on normalised features (n x 6, FP32) and
+/* One forward/backward pass of `member` on normalised features (n x 8, FP32) and
  * scaled targets (n, FP32), both host memory, n <= 131072: returns the mean squared
  * error and its gradient with respect to every parameter (host, KWGPU_SURROGATE_PARAMS
  * values). */
@@ -285,11 +323,13 @@ int kwgpu_surrogate_loss_grad(kwgpu_ctx* ctx, kwgpu_surrogate* s, int32_t member
 int kwgpu_surrogate_baseline(kwgpu_ctx* ctx, const kwgpu_option* host_opts, size_t n,
                              double* host_out);
 
-/* Normalised features (n x 6) of options, host memory, as the network sees them. */
+/* Normalised features (n x 8) of options priced on an x_dim x t_dim grid, host memory,
+ * as the network sees them. */
 int kwgpu_surrogate_features(kwgpu_ctx* ctx, const kwgpu_option* host_opts, size_t n,
-                             float* host_features);
+                             int32_t x_dim, int32_t t_dim, float* host_features);
 
-/* Saves / loads an ensemble (a small binary file: magic, members, FP32 weights). */
+/* Saves / loads an ensemble (a small binary file: magic, version, members, the FD
+ * configuration it emulates, FP32 weights). */
 int kwgpu_surrogate_save(kwgpu_ctx* ctx, const kwgpu_surrogate* s, const char* path);
 int kwgpu_surrogate_load(kwgpu_ctx* ctx, const char* path, kwgpu_surrogate** out);
 

@@ -268,11 +268,14 @@ __device__ __forceinline__ Map2 select_map(bool p, const Map2& x, const Map2& y)
 // l_j = bl_j / d_{j-1}, u_j = bu_j / d_{j+1}, R/Q the carry fix-ups, blv/buv the
 // off-diagonals of B, and sm the scan multipliers. Every cross-lane step below reads
 // only lanes of the same option, or discards what it read with a select.
-template <int M, int L, int W>
+// TF is the type the march takes the factors, fix-ups and scan multipliers in: double
+// for the FP64 march; float for the FP32-mixed one, which gets each as it is produced
+// (so no FP64 copy of the factor arrays stays live beside the FP32 one).
+template <int M, int L, int W, class TF>
 __device__ __forceinline__ void setup(const FdBatch& b, const Pde& d, const Lanes& t, double* qp,
-                                      double (&xv)[M], double (&ap)[M], double (&lo)[M],
-                                      double (&up)[M], double (&R)[M], double (&Q)[M],
-                                      double (&blv)[M], double (&buv)[M], ScanMul<double>& sm) {
+                                      double (&xv)[M], TF (&ap)[M], TF (&lo)[M],
+                                      TF (&up)[M], TF (&R)[M], TF (&Q)[M],
+                                      double (&blv)[M], double (&buv)[M], ScanMul<TF>& sm) {
     using Gm = Geo<M, L, W>;
     constexpr int LT = Gm::LT;
     const PdeParams& P = d.P;
@@ -364,10 +367,10 @@ __device__ __forceinline__ void setup(const FdBatch& b, const Pde& d, const Lane
             // Thomas pivots of the reference (gam_j = bu_{j-1}/bet, bet = b_j - bl_j gam_j)
             const double dd = bv[i] - blv[i] * (bup * rprev);
             const double rd = rcp(dd);
-            lo[i] = blv[i] * rprev;  // bl = 0 on row 0, whatever the entering pivot
+            lo[i] = (TF)(blv[i] * rprev);  // bl = 0 on row 0, whatever the entering pivot
             if (i == 0) d0 = dd;
-            if (i > 0) up[i - 1] = buv[i - 1] * rd;  // u_j = bu_j / d_{j+1}
-            ap[i] = inv_th * rd;
+            if (i > 0) up[i - 1] = (TF)(buv[i - 1] * rd);  // u_j = bu_j / d_{j+1}
+            ap[i] = (TF)(inv_th * rd);
             rprev = rd;
             bup = buv[i];
         }
@@ -377,24 +380,28 @@ __device__ __forceinline__ void setup(const FdBatch& b, const Pde& d, const Lane
     __syncthreads();
     qp[gl] = d0;
     __syncthreads();
-    up[M - 1] = buv[M - 1] * (t.ls + 1 < LT ? rcp(qp[gl + 1]) : 1.0);
+    up[M - 1] = (TF)(buv[M - 1] * (t.ls + 1 < LT ? rcp(qp[gl + 1]) : 1.0));
 
     // Carry propagation coefficients: g_j = ghat_j + Pc_j g_in,
     // w_j = what_j + R_j g_in + Q_j w_in.
     double Pl, Ql;
     {
         double Pc[M];
-        Pc[0] = -lo[0];
+        Pc[0] = -(double)lo[0];
 #pragma unroll
-        for (int i = 1; i < M; ++i) Pc[i] = -lo[i] * Pc[i - 1];
-        R[M - 1] = Pc[M - 1];
+        for (int i = 1; i < M; ++i) Pc[i] = -(double)lo[i] * Pc[i - 1];
+        double r = Pc[M - 1], q = -(double)up[M - 1];
+        R[M - 1] = (TF)r;
+        Q[M - 1] = (TF)q;
 #pragma unroll
-        for (int i = M - 2; i >= 0; --i) R[i] = Pc[i] - up[i] * R[i + 1];
-        Q[M - 1] = -up[M - 1];
-#pragma unroll
-        for (int i = M - 2; i >= 0; --i) Q[i] = -up[i] * Q[i + 1];
+        for (int i = M - 2; i >= 0; --i) {
+            r = Pc[i] - (double)up[i] * r;
+            q = -(double)up[i] * q;
+            R[i] = (TF)r;
+            Q[i] = (TF)q;
+        }
         Pl = Pc[M - 1];
-        Ql = Q[0];
+        Ql = q;
     }
     // Scan multipliers: products of the lanes' Pl (forward) and Ql (backward) over the
     // windows each scan stage spans. Lanes whose window leaves the row take 0 by select.
@@ -402,17 +409,17 @@ __device__ __forceinline__ void setup(const FdBatch& b, const Pde& d, const Lane
         const double w2 = Pl * dpp<kRowShr + 1>(Pl);  // Pl over lanes l-1 .. l
         const double w4 = w2 * dpp<kRowShr + 2>(w2);  // l-3 .. l
         const double w8 = w4 * dpp<kRowShr + 4>(w4);  // l-7 .. l
-        sm.F1 = pr >= 1 ? Pl : 0.0;
-        sm.F2 = pr >= 2 ? w2 : 0.0;
-        sm.F4 = pr >= 4 ? w4 : 0.0;
-        sm.F8 = pr >= 8 ? w8 : 0.0;
+        sm.F1 = (TF)(pr >= 1 ? Pl : 0.0);
+        sm.F2 = (TF)(pr >= 2 ? w2 : 0.0);
+        sm.F4 = (TF)(pr >= 4 ? w4 : 0.0);
+        sm.F8 = (TF)(pr >= 8 ? w8 : 0.0);
         const double v2 = Ql * dpp<kRowShl + 1>(Ql);  // Ql over lanes l .. l+1
         const double v4 = v2 * dpp<kRowShl + 2>(v2);  // l .. l+3
         const double v8 = v4 * dpp<kRowShl + 4>(v4);  // l .. l+7
-        sm.G1 = pr + 1 <= 15 ? Ql : 0.0;
-        sm.G2 = pr + 2 <= 15 ? v2 : 0.0;
-        sm.G4 = pr + 4 <= 15 ? v4 : 0.0;
-        sm.G8 = pr + 8 <= 15 ? v8 : 0.0;
+        sm.G1 = (TF)(pr + 1 <= 15 ? Ql : 0.0);
+        sm.G2 = (TF)(pr + 2 <= 15 ? v2 : 0.0);
+        sm.G4 = (TF)(pr + 4 <= 15 ? v4 : 0.0);
+        sm.G8 = (TF)(pr + 8 <= 15 ? v8 : 0.0);
         // row-inclusive prefix of Pl and suffix of Ql
         double ip = Pl, sq = Ql;
         {
@@ -430,22 +437,22 @@ __device__ __forceinline__ void setup(const FdBatch& b, const Pde& d, const Lane
         // Ql over rows 1..3 from the suffix at each row's start
         const double rp0 = readlane<15>(ip), rp1 = readlane<31>(ip), rp2 = readlane<47>(ip);
         const double rs1 = readlane<16>(sq), rs2 = readlane<32>(sq), rs3 = readlane<48>(sq);
-        sm.FB15 = (L >= 32 && (wrow & 1)) ? ip : 0.0;
-        sm.FB31 = (L == 64 && wrow >= 2) ? (wrow == 3 ? ip * rp2 : ip) : 0.0;
+        sm.FB15 = (TF)((L >= 32 && (wrow & 1)) ? ip : 0.0);
+        sm.FB31 = (TF)((L == 64 && wrow >= 2) ? (wrow == 3 ? ip * rp2 : ip) : 0.0);
         const bool w64 = L == 64, w32 = L == 32;
-        sm.X16 = (w64 && wrow == 0) || (w32 && (wrow & 1) == 0) ? sq : 0.0;
-        sm.X32 = w64 && wrow == 1 ? sq : 0.0;
-        sm.X48 = w64 && wrow == 2 ? sq : 0.0;
+        sm.X16 = (TF)((w64 && wrow == 0) || (w32 && (wrow & 1) == 0) ? sq : 0.0);
+        sm.X32 = (TF)(w64 && wrow == 1 ? sq : 0.0);
+        sm.X48 = (TF)(w64 && wrow == 2 ? sq : 0.0);
         // two-wave groups: Pl over wave 1's lanes 0..l (carry of wave 0's last lane into
         // wave 1) and Ql over wave 0's lanes l..63 (carry of wave 1's first lane back)
         if constexpr (W == 2) {
             const double before = wrow == 0 ? 1.0 : wrow == 1 ? rp0 : wrow == 2 ? rp0 * rp1 : rp0 * rp1 * rp2;
             const double after = wrow == 3 ? 1.0 : wrow == 2 ? rs3 : wrow == 1 ? rs2 * rs3 : rs1 * rs2 * rs3;
-            sm.FW = wv == 1 ? ip * before : 0.0;
-            sm.ZW = wv == 0 ? sq * after : 0.0;
+            sm.FW = (TF)(wv == 1 ? ip * before : 0.0);
+            sm.ZW = (TF)(wv == 0 ? sq * after : 0.0);
         } else {
-            sm.FW = 0.0;
-            sm.ZW = 0.0;
+            sm.FW = TF(0);
+            sm.ZW = TF(0);
         }
     }
 }
@@ -797,25 +804,7 @@ __device__ __forceinline__ void solve_group_mixed(const FdBatch& b, int group, d
     double xv[M], blv[M], buv[M];
     float ap[M], lo[M], up[M], R[M], Q[M];
     ScanMul<float> sm;
-    {
-        double ap64[M], lo64[M], up64[M], R64[M], Q64[M];
-        ScanMul<double> s64;
-        setup<M, L, W>(b, d, t, qp, xv, ap64, lo64, up64, R64, Q64, blv, buv, s64);
-#pragma unroll
-        for (int i = 0; i < M; ++i) {
-            ap[i] = (float)ap64[i];
-            lo[i] = (float)lo64[i];
-            up[i] = (float)up64[i];
-            R[i] = (float)R64[i];
-            Q[i] = (float)Q64[i];
-        }
-        sm.F1 = (float)s64.F1; sm.F2 = (float)s64.F2; sm.F4 = (float)s64.F4;
-        sm.F8 = (float)s64.F8; sm.FB15 = (float)s64.FB15; sm.FB31 = (float)s64.FB31;
-        sm.G1 = (float)s64.G1; sm.G2 = (float)s64.G2; sm.G4 = (float)s64.G4;
-        sm.G8 = (float)s64.G8;
-        sm.X16 = (float)s64.X16; sm.X32 = (float)s64.X32; sm.X48 = (float)s64.X48; sm.FW = (float)s64.FW;
-        sm.ZW = (float)s64.ZW;
-    }
+    setup<M, L, W>(b, d, t, qp, xv, ap, lo, up, R, Q, blv, buv, sm);
     // diagonal of (I - B) is theta dt a0 on every real row (the stencils' weights sum
     // to zero); padding rows hold V = 0
     const double sdiag = b.theta * d.P.dt * d.P.a0;

@@ -44,6 +44,7 @@ struct Bucket {
 constexpr int kLaneCap = 1 << 30;
 constexpr int kSide = 4;  // side streams an uneven batch forks onto
 
+
 struct kwgpu_ctx {
     int device = 0;
     hipStream_t stream = nullptr;

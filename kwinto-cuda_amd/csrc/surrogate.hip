@@ -1,13 +1,20 @@
 // surrogate.hip -- learned early-exercise premium in front of the FD pricer.
 //
-// An ensemble of MLPs 6 -> 256 -> 256 -> 256 -> 256 -> 1 (SiLU) predicting
-// y = asinh(premium / (K s0)) / c, premium = FD American price - BS European price,
-// s0 = 1e-5, c = 4, from normalised features. The signed-log target makes the network's
-// error relative for large premiums and about K s0 |dy| for small ones, so options whose
-// premium is small can be served to a tight absolute error.
+// The surrogate emulates one FD pricer configuration (density, scale, theta and either
+// one x/t grid or per-option grids). An ensemble of MLPs 8 -> 256 -> 256 -> 256 -> 256
+// -> 1 (SiLU) predicts, at a mesh node x_j of an option's own FD grid, the FD solution
+// there minus the Black-Scholes price, as y = (premium - baw) / (baw + 1e-4 K) against
+// the Barone-Adesi & Whaley approximation baw. A price is then formed exactly as the FD
+// pricer forms its own (Fd1d::value, src/Math/kwFd1d.cpp:139-158): the network is
+// evaluated at the two nodes that bracket log(S/K), BS is added at each node, and the
+// two node values are interpolated linearly with the reference's weights. The
+// interpolation error between nodes -- up to ~1e-4 of price, and oscillating with the
+// position of log(S/K) in its cell -- is then reproduced rather than learned.
+// Labels are the FD pricer's own values at nodes: an option with spot K e^{x_j} is
+// priced at node j (interpolation weight 1).
 // The three 256x256 layers run forward and backward on BF16 MFMA
 // (v_mfma_f32_16x16x32_bf16, FP32 accumulation) through one strided GEMM kernel with
-// fused epilogues; the 6-wide input and 1-wide output layers are FP32; Adam keeps FP32
+// fused epilogues; the 8-wide input and 1-wide output layers are FP32; Adam keeps FP32
 // master weights and refreshes the BF16 copies after every step. The reference has no
 // counterpart: the labels come from this build's FD pricer, which is also the fallback.
 #include <math.h>
@@ -18,6 +25,7 @@
 #include <numeric>
 #include <random>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "kwgpu_internal.h"
@@ -30,7 +38,8 @@ typedef bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 constexpr int H = 256;  // hidden width
-constexpr int F = 6;    // features
+constexpr int F = 8;    // features: log(S/K), T, sigma, r, q, w, then the FD grid's x and t points
+constexpr int F0 = 6;   // ... of which the option's own (the training box applies to these)
 constexpr int kW1 = 0, kB1 = kW1 + H * F, kW2 = kB1 + H, kB2 = kW2 + H * H, kW3 = kB2 + H,
               kB3 = kW3 + H * H, kW4 = kB3 + H, kB4 = kW4 + H * H, kW5 = kB4 + H, kB5 = kW5 + H,
               kP = kB5 + 1;
@@ -47,7 +56,7 @@ constexpr size_t kPartFloats = std::max<size_t>((size_t)kPartRows * kPartWidth,
 // (floored at kEps K). A network error dy then costs (baw + kEps K) dy in price: small
 // where the premium is small, which is where an absolute error bar can be met.
 constexpr double kEps = 1e-4;
-constexpr uint32_t kFileVersion = 3;  // saved parameters belong to this target
+constexpr uint32_t kFileVersion = 4;  // saved parameters belong to this target and features
 
 // Barone-Adesi & Whaley (1987) quadratic approximation of the early-exercise premium
 // (American minus European Black-Scholes) with a continuous dividend yield q. With
@@ -101,7 +110,7 @@ __host__ __device__ inline double premium_of(double y, double k, double baw) {
 
 // Training box, as (centre, half-width) of each raw feature.
 constexpr double kLogMin = -0.69314718055994530942, kLogMax = 0.91629073187415506518;  // K/S 2, 0.4
-constexpr double kBox[F][2] = {
+constexpr double kBox[F0][2] = {
     {(kLogMin + kLogMax) / 2, (kLogMax - kLogMin) / 2},  // log(S/K)
     {(1.0 / 52 + 5.0) / 2, (5.0 - 1.0 / 52) / 2},        // T
     {0.35, 0.25},                                          // sigma
@@ -110,15 +119,55 @@ constexpr double kBox[F][2] = {
     {0.0, 1.0},                                            // w
 };
 
-__host__ __device__ inline bool option_features(const kwgpu_option& o, float* f) {
-    const double raw[F] = {log(o.s / o.k), o.t, o.z, o.r, o.q, o.w < 0 ? -1.0 : 1.0};
+// Normalised features of an option priced on an x_dim x t_dim grid; true when the
+// option lies in the training box (American, K/S, T, sigma, r, q inside their ranges).
+// The grid enters as log2 of its point counts (x 256 .. 2048 and t 32 .. 2048 map to
+// [-1, 1]); a surrogate of a single grid sees constants there.
+__host__ __device__ inline bool option_features(const kwgpu_option& o, int x_dim, int t_dim,
+                                                float* f) {
+    const double raw[F0] = {log(o.s / o.k), o.t, o.z, o.r, o.q, o.w < 0 ? -1.0 : 1.0};
     bool in = o.e != 0 && o.s > 0 && o.k > 0;
-    for (int i = 0; i < F; ++i) {
+    for (int i = 0; i < F0; ++i) {
         const double u = (raw[i] - kBox[i][0]) / kBox[i][1];
         f[i] = (float)u;
         in = in && (u >= -1.0 - 1e-12) && (u <= 1.0 + 1e-12);
     }
+    f[F0] = (float)((log2((double)x_dim) - 9.5) / 1.5);
+    f[F0 + 1] = (float)((log2((double)t_dim) - 8.0) / 3.0);
     return in;
+}
+
+// The two mesh nodes that bracket log(S/K) on the option's grid (Fd1d::value's
+// bracket: the first node x_j >= log(S/K)), as options at those spots, with the
+// interpolation abscissae. ok is false where the network has no labels to stand on:
+// an invalid option, a spot outside the grid, or a bracket at the first cell (the FD
+// pricer rejects a spot at x_0, so no label exists there).
+struct NodePair {
+    kwgpu_option a, b;  // spots K e^{x_{j-1}}, K e^{x_j}
+    double x0, x1, xs;
+    bool ok;
+};
+
+__device__ inline NodePair node_pair(const kwgpu_option& o, int x_dim, int t_dim, double density,
+                                     double scale) {
+    NodePair q;
+    const PdeParams P = pde_params(o, x_dim, t_dim, density, scale);
+    q.xs = log(o.s / o.k);
+    const int j = P.valid && isfinite(q.xs) ? bracket(P, density, q.xs) : 0;
+    q.ok = P.valid && j >= 2 && j < x_dim;
+    const int jj = q.ok ? j : 2;
+    q.x0 = grid_x(P, density, jj - 1);
+    q.x1 = grid_x(P, density, jj);
+    q.a = o;
+    q.b = o;
+    q.a.s = o.k * exp(q.x0);
+    q.b.s = o.k * exp(q.x1);
+    return q;
+}
+
+__device__ inline double euro_bs(kwgpu_option o) {
+    o.e = 0;
+    return bs_one(o);
 }
 
 __device__ __forceinline__ float sigm(float z) { return 1.0f / (1.0f + __expf(-z)); }
@@ -129,13 +178,23 @@ __device__ __forceinline__ float dsilu(float z) {
 }
 
 // ---- data ------------------------------------------------------------------------
-__global__ void features_kernel(const kwgpu_option* opts, size_t n, const double* premium,
+// Grid of row i: per-row sizes when given, else the uniform ones.
+struct GridArg {
+    const int32_t* x_dims;
+    const int32_t* t_dims;
+    int x_dim, t_dim;
+    double density, scale;
+    __device__ int x(size_t i) const { return x_dims ? x_dims[i] : x_dim; }
+    __device__ int t(size_t i) const { return t_dims ? t_dims[i] : t_dim; }
+};
+
+__global__ void features_kernel(const kwgpu_option* opts, size_t n, GridArg g, const double* premium,
                                 float* feat, float* target, uint8_t* in_box) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const kwgpu_option o = opts[i];
     float f[F];
-    const bool in = option_features(o, f);
+    const bool in = option_features(o, g.x(i), g.t(i), f);
     for (int k = 0; k < F; ++k) feat[i * F + k] = f[k];
     if (target) {
         const double baw = baw_premium(o);
@@ -156,6 +215,84 @@ __global__ void gather_kernel(const float* feat, const float* target, const uint
     const uint32_t s = idx[k * sidx + i];
     for (int f = 0; f < F; ++f) bfeat[k * srow * F + i * F + f] = feat[(size_t)s * F + f];
     btarget[k * srow + i] = target[s];
+}
+
+// Labels at nodes: option i gives node options a_i (row i) and b_i (row n + i) with the
+// option's grid sizes; ok[i] = 0 where node_pair has no labelled bracket.
+__global__ void node_options_kernel(const kwgpu_option* opts, size_t n, GridArg g,
+                                    kwgpu_option* nodes, int32_t* nx, int32_t* nt, uint8_t* ok) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const NodePair q = node_pair(opts[i], g.x(i), g.t(i), g.density, g.scale);
+    nodes[i] = q.a;
+    nodes[n + i] = q.b;
+    if (nx) nx[i] = nx[n + i] = g.x(i);
+    if (nt) nt[i] = nt[n + i] = g.t(i);
+    ok[i] = q.ok ? 1 : 0;
+}
+
+// premium = FD price - BS European price at each node of pair i; keep[i] when the pair
+// has a labelled bracket and both nodes priced to a finite premium
+__global__ void node_premium_kernel(const kwgpu_option* nodes, size_t n, const double* fd,
+                                    const int32_t* st, const uint8_t* ok, double* premium,
+                                    uint8_t* keep) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double pa = fd[i] - euro_bs(nodes[i]);
+    const double pb = fd[n + i] - euro_bs(nodes[n + i]);
+    premium[i] = pa;
+    premium[n + i] = pb;
+    keep[i] = ok[i] && st[i] == KWGPU_STATUS_OK && st[n + i] == KWGPU_STATUS_OK && isfinite(pa) &&
+                      isfinite(pb)
+                  ? 1
+                  : 0;
+}
+
+// Prediction rows: node a of option i in row i, node b in row cn + i; in_box[i] when the
+// option is in the training box and has a labelled bracket.
+__global__ void node_features_kernel(const kwgpu_option* opts, size_t cn, GridArg g, float* feat,
+                                     uint8_t* in_box) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cn) return;
+    const kwgpu_option o = opts[i];
+    const int xd = g.x(i), td = g.t(i);
+    const NodePair q = node_pair(o, xd, td, g.density, g.scale);
+    float f[F];
+    const bool in = option_features(o, xd, td, f);
+    option_features(q.a, xd, td, feat + i * F);
+    option_features(q.b, xd, td, feat + (cn + i) * F);
+    if (in_box) in_box[i] = in && q.ok ? 1 : 0;
+}
+
+// Ensemble combine in price units, as the FD pricer interpolates: member m's node values
+// v = BS + premium at x_{j-1} and x_j, its price ((x_j - x) v_{j-1} + (x - x_{j-1}) v_j) /
+// (x_j - x_{j-1}); premium[i] = mean price - BS at the spot, spread[i] = max - min price.
+__global__ void node_combine_kernel(const kwgpu_option* opts, size_t cn, GridArg g,
+                                    const float* y, int members, size_t ld, double* premium,
+                                    double* spread) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cn) return;
+    const kwgpu_option o = opts[i];
+    const NodePair q = node_pair(o, g.x(i), g.t(i), g.density, g.scale);
+    if (!q.ok) {
+        premium[i] = NAN;
+        spread[i] = INFINITY;
+        return;
+    }
+    const double bwa = baw_premium(q.a), bwb = baw_premium(q.b);
+    const double bsa = euro_bs(q.a), bsb = euro_bs(q.b);
+    const double ua = q.x1 - q.xs, ub = q.xs - q.x0, h = q.x1 - q.x0;
+    double sum = 0, lo = 1e300, hi = -1e300;
+    for (int m = 0; m < members; ++m) {
+        const double va = bsa + premium_of(y[m * ld + i], o.k, bwa);
+        const double vb = bsb + premium_of(y[m * ld + cn + i], o.k, bwb);
+        const double v = (ua * va + ub * vb) / h;
+        sum += v;
+        lo = fmin(lo, v);
+        hi = fmax(hi, v);
+    }
+    premium[i] = sum / members - euro_bs(o);
+    spread[i] = hi - lo;
 }
 
 __global__ void baseline_kernel(const kwgpu_option* opts, size_t n, double* out) {
@@ -550,24 +687,6 @@ __global__ void zero_f(float* p, size_t n) {
     if (i < n) p[i] = 0.0f;
 }
 
-// ensemble combine in price units: premium = mean over members, spread = max - min
-__global__ void combine_kernel(const kwgpu_option* opts, const float* y, int members, size_t n,
-                               size_t ld, double* premium, double* spread) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const double k = opts[i].k;
-    const double baw = baw_premium(opts[i]);
-    double s = 0, lo = 1e300, hi = -1e300;
-    for (int m = 0; m < members; ++m) {
-        const double v = premium_of(y[m * ld + i], k, baw);
-        s += v;
-        lo = fmin(lo, v);
-        hi = fmax(hi, v);
-    }
-    premium[i] = s / members;
-    spread[i] = hi - lo;
-}
-
 // ---- serving ---------------------------------------------------------------------
 // Route 0: surrogate (in the box, ensemble spread <= spread_max): price = BS + premium;
 // premium == nullptr (no prediction made) routes every option to FD.
@@ -721,6 +840,15 @@ struct kwgpu_surrogate {
                             // blocks x kPartWidth per member)
     void* serve = nullptr;  // kwgpu_surrogate_price's per-option buffers (grown on demand)
     size_t serve_cap = 0;   // options they hold
+    // the FD configuration the network emulates (set by kwgpu_surrogate_fit; by default
+    // the 1024 x 1024 reference defaults), and whether its grids are per option
+    double density = 0.25, scale = 50.0, theta = 0.5;
+    int x_dim = 1024, t_dim = 1024;
+    bool per_option = false;
+    // end of the last prediction and its stream: a prediction on another stream waits
+    // for it before it reuses the scratch buffers
+    hipEvent_t ev_done = nullptr;
+    hipStream_t done_stream = nullptr;
 };
 
 namespace {
@@ -734,6 +862,28 @@ namespace {
     } while (0)
 
 unsigned nblk(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
+
+// Does `fd` describe the FD configuration the surrogate emulates? (Same density, scale
+// and theta; the same single grid, or per-option grids for a per-option surrogate.)
+bool grid_matches(const kwgpu_surrogate* s, const kwgpu_fd1d_params* fd) {
+    if (!fd || fd->density != s->density || fd->scale != s->scale || fd->theta != s->theta)
+        return false;
+    if (s->per_option) return fd->x_dims && fd->t_dims;
+    return !fd->x_dims && !fd->t_dims && fd->x_dim == s->x_dim && fd->t_dim == s->t_dim;
+}
+
+GridArg grid_arg(const kwgpu_surrogate* s, const kwgpu_fd1d_params* fd) {
+    GridArg g{};
+    g.density = s->density;
+    g.scale = s->scale;
+    g.x_dim = s->x_dim;
+    g.t_dim = s->t_dim;
+    if (fd && s->per_option) {
+        g.x_dims = fd->x_dims;
+        g.t_dims = fd->t_dims;
+    }
+    return g;
+}
 
 void free_scratch(kwgpu_surrogate* s) {
     for (void* q : {(void*)s->x, (void*)s->tg, (void*)s->y, (void*)s->dy, (void*)s->Z,
@@ -927,6 +1077,10 @@ int kwgpu_surrogate_create(kwgpu_ctx* ctx, int32_t members, uint64_t seed, kwgpu
     auto* s = new kwgpu_surrogate();
     s->members = members;
     s->device = kwgpu::ctx_device(ctx);
+    if (hipEventCreateWithFlags(&s->ev_done, hipEventDisableTiming)) {
+        delete s;
+        return kwgpu::ctx_fail(ctx, KWGPU_ERR_HIP, "kwgpu_surrogate_create: event");
+    }
     std::vector<float> w;
     hipStream_t st = kwgpu::ctx_stream(ctx);
     const size_t M = (size_t)members;
@@ -966,12 +1120,15 @@ void kwgpu_surrogate_destroy(kwgpu_surrogate* s) {
                     (void*)s->m1, (void*)s->m2,
                     (void*)s->loss, (void*)s->yall, (void*)s->part, s->serve})
         if (q) (void)hipFree(q);
+    if (s->ev_done) (void)hipEventDestroy(s->ev_done);
     delete s;
 }
 
-int kwgpu_surrogate_features(kwgpu_ctx* ctx, const kwgpu_option* opts, size_t n, float* feat) {
-    if (!opts || !feat) return kwgpu::ctx_fail(ctx, KWGPU_ERR_INVALID, "null pointer");
-    for (size_t i = 0; i < n; ++i) (void)option_features(opts[i], feat + i * F);
+int kwgpu_surrogate_features(kwgpu_ctx* ctx, const kwgpu_option* opts, size_t n, int32_t x_dim,
+                             int32_t t_dim, float* feat) {
+    if (!opts || !feat || x_dim < 3 || t_dim < 2)
+        return kwgpu::ctx_fail(ctx, KWGPU_ERR_INVALID, "kwgpu_surrogate_features: bad argument");
+    for (size_t i = 0; i < n; ++i) (void)option_features(opts[i], x_dim, t_dim, feat + i * F);
     return KWGPU_OK;
 }
 
@@ -1041,122 +1198,300 @@ int kwgpu_surrogate_loss_grad(kwgpu_ctx* ctx, kwgpu_surrogate* s, int32_t member
     return KWGPU_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Trains every member on rows (option, premium) held on the device, row i's features
+// taken on grid g; `rows` (host, optional) lists the rows to use, all n otherwise. Member
+// m shuffles with its own stream seeded from tp->seed; the next epoch's permutation is
+// drawn on a host thread while the current epoch runs, into the other of two pinned /
+// device buffer pairs.
+int train_rows(kwgpu_ctx* ctx, kwgpu_surrogate* s, const kwgpu_option* d_opts,
+               const double* d_premium, size_t n, const GridArg& g,
+               const std::vector<uint32_t>* rows, const kwgpu_surrogate_train_params* tp,
+               float* loss_out) {
+    hipStream_t st = kwgpu::ctx_stream(ctx);
+    const int B = tp->batch;
+    const int M = s->members;
+    const size_t nr = rows ? rows->size() : n;
+    if (nr < (size_t)B)
+        return kwgpu::ctx_fail(ctx, KWGPU_ERR_INVALID,
+                               "kwgpu_surrogate_train: fewer usable rows (" + std::to_string(nr) +
+                                   ") than a batch");
+    float *feat = nullptr, *tgt = nullptr;
+    uint32_t* dperm[2] = {nullptr, nullptr};
+    uint32_t* hperm[2] = {nullptr, nullptr};
+    int rc = KWGPU_OK;
+    auto cleanup = [&]() {
+        (void)hipStreamSynchronize(st);
+        (void)hipFree(feat);
+        (void)hipFree(tgt);
+        for (int k = 0; k < 2; ++k) {
+            (void)hipFree(dperm[k]);
+            (void)hipHostFree(hperm[k]);
+        }
+    };
+    const size_t pbytes = (size_t)M * nr * sizeof(uint32_t);
+    if (hipMalloc(&feat, n * F * sizeof(float)) || hipMalloc(&tgt, n * sizeof(float)) ||
+        hipMalloc(&dperm[0], pbytes) || hipMalloc(&dperm[1], pbytes) ||
+        hipHostMalloc((void**)&hperm[0], pbytes, hipHostMallocDefault) ||
+        hipHostMalloc((void**)&hperm[1], pbytes, hipHostMallocDefault) || ensure_scratch(s, B)) {
+        cleanup();
+        return kwgpu::ctx_fail(ctx, KWGPU_ERR_NOMEM, "kwgpu_surrogate_train: allocation");
+    }
+    hipLaunchKernelGGL(features_kernel, dim3(nblk(n, 256)), dim3(256), 0, st, d_opts, n, g,
+                       d_premium, feat, tgt, (uint8_t*)nullptr);
+    // every member steps at once (one launch per layer for the whole ensemble), each on
+    // its own shuffle of the rows
+    std::vector<std::mt19937_64> rng;
+    for (int m = 0; m < M; ++m) rng.emplace_back(tp->seed + 7919ull * (uint64_t)m);
+    auto draw = [&](uint32_t* dst) {
+        for (int m = 0; m < M; ++m) {
+            uint32_t* first = dst + (size_t)m * nr;
+            if (rows)
+                std::copy(rows->begin(), rows->end(), first);
+            else
+                std::iota(first, first + nr, 0u);
+            std::shuffle(first, first + nr, rng[m]);
+        }
+    };
+    const size_t steps_per_epoch = nr / B;  // drop the ragged tail
+    const double total = (double)steps_per_epoch * tp->epochs;
+    const long cap = (long)s->cap;
+    hipLaunchKernelGGL(zero_f, dim3(nblk((size_t)M * kP, 256)), dim3(256), 0, st, s->m1,
+                       (size_t)M * kP);
+    hipLaunchKernelGGL(zero_f, dim3(nblk((size_t)M * kP, 256)), dim3(256), 0, st, s->m2,
+                       (size_t)M * kP);
+    draw(hperm[0]);
+    if (hipError_t e = hipMemcpyAsync(dperm[0], hperm[0], pbytes, hipMemcpyHostToDevice, st)) {
+        cleanup();
+        return kwgpu::ctx_fail(ctx, KWGPU_ERR_HIP, std::string("permutation: ") + hipGetErrorString(e));
+    }
+    long step = 0;
+    std::vector<float> lsum(M);
+    for (int ep = 0; ep < tp->epochs && rc == KWGPU_OK; ++ep) {
+        const uint32_t* didx = dperm[ep & 1];
+        // the next epoch's order, drawn while this one runs: its host buffer was last
+        // copied from before the previous epoch's end, which was waited for
+        std::thread next;
+        if (ep + 1 < tp->epochs) next = std::thread(draw, hperm[(ep + 1) & 1]);
+        hipLaunchKernelGGL(zero_f, dim3(1), dim3(64), 0, st, s->loss, (size_t)M);
+        for (size_t b = 0; b < steps_per_epoch; ++b, ++step) {
+            hipLaunchKernelGGL(gather_kernel, dim3(nblk(B, 256), M), dim3(256), 0, st, feat, tgt,
+                               didx + b * B, (long)nr, B, s->x, s->tg, cap);
+            hipError_t e = forward(s, 0, M, B, false, s->y, cap, true, st);
+            if (!e) e = backward(s, 0, M, B, st);
+            const double frac = step / total;
+            const float lr = (float)(tp->lr_final + 0.5 * (tp->lr - tp->lr_final) *
+                                                         (1.0 + std::cos(M_PI * frac)));
+            const float c1 = 1.0f - std::pow(tp->beta1, (float)(step + 1));
+            const float c2 = 1.0f - std::pow(tp->beta2, (float)(step + 1));
+            hipLaunchKernelGGL(adam_kernel, dim3(nblk(kP, 256), M), dim3(256), 0, st, s->pall,
+                               s->grad, s->m1, s->m2, kP, lr, tp->beta1, tp->beta2, tp->eps, c1,
+                               c2);
+            if (!e) e = refresh_bf16(s, 0, M, st);
+            if (e) {
+                rc = kwgpu::ctx_fail(ctx, KWGPU_ERR_HIP,
+                                     std::string("train step: ") + hipGetErrorString(e));
+                break;
+            }
+        }
+        if (next.joinable()) next.join();
+        if (rc == KWGPU_OK && ep + 1 < tp->epochs) {
+            // the other device buffer was last read by the previous epoch (stream order)
+            if (hipError_t e = hipMemcpyAsync(dperm[(ep + 1) & 1], hperm[(ep + 1) & 1], pbytes,
+                                              hipMemcpyHostToDevice, st))
+                rc = kwgpu::ctx_fail(ctx, KWGPU_ERR_HIP, hipGetErrorString(e));
+        }
+        if (hipError_t e = hipStreamSynchronize(st)) {
+            if (rc == KWGPU_OK) rc = kwgpu::ctx_fail(ctx, KWGPU_ERR_HIP, hipGetErrorString(e));
+            break;
+        }
+        (void)hipMemcpy(lsum.data(), s->loss, M * sizeof(float), hipMemcpyDeviceToHost);
+        if (loss_out)
+            for (int m = 0; m < M; ++m)
+                loss_out[(size_t)ep * M + m] = lsum[m] / (float)(steps_per_epoch * B);
+    }
+    cleanup();
+    return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
 int kwgpu_surrogate_train(kwgpu_ctx* ctx, kwgpu_surrogate* s, const kwgpu_option* d_opts,
                           const double* d_premium, size_t n, const kwgpu_surrogate_train_params* tp,
                           float* loss_out) {
     if (!s || !d_opts || !d_premium || !tp || n == 0 || tp->batch < 64 || tp->epochs < 1 ||
         n > 0xffffffffu)
         return kwgpu::ctx_fail(ctx, KWGPU_ERR_INVALID, "kwgpu_surrogate_train: bad argument");
+    if (s->per_option)
+        return kwgpu::ctx_fail(ctx, KWGPU_ERR_INVALID,
+                               "kwgpu_surrogate_train: the surrogate emulates per-option grids; "
+                               "use kwgpu_surrogate_fit");
+    HIPCHK(hipSetDevice(s->device));
+    return train_rows(ctx, s, d_opts, d_premium, n, grid_arg(s, nullptr), nullptr, tp, loss_out);
+}
+
+int kwgpu_surrogate_labels(kwgpu_ctx* ctx, const kwgpu_option* d_opts, size_t n,
+                           const kwgpu_fd1d_params* fd, kwgpu_option* d_nodes, double* d_premium,
+                           uint8_t* d_keep) {
+    if (!d_opts || !fd || !d_nodes || !d_premium || !d_keep || n == 0 || n > 0x3fffffffu ||
+        (!fd->x_dims) != (!fd->t_dims))
+        return kwgpu::ctx_fail(ctx, KWGPU_ERR_INVALID, "kwgpu_surrogate_labels: bad argument");
+    if (fd->precision != KWGPU_FP64)
+        return kwgpu::ctx_fail(ctx, KWGPU_ERR_INVALID, "kwgpu_surrogate_labels: labels are FP64 FD prices");
+    HIPCHK(hipSetDevice(kwgpu::ctx_device(ctx)));
+    hipStream_t st = kwgpu::ctx_stream(ctx);
+    const size_t n2 = 2 * n;
+    int32_t *nx = nullptr, *nt = nullptr, *fst = nullptr;
+    double* fdp = nullptr;
+    uint8_t* ok = nullptr;
+    auto cleanup = [&]() {
+        (void)hipStreamSynchronize(st);
+        for (void* q : {(void*)nx, (void*)nt, (void*)fst, (void*)fdp, (void*)ok}) (void)hipFree(q);
+    };
+    const bool per_option = fd->x_dims != nullptr;
+    if (hipMalloc(&fst, n2 * sizeof(int32_t)) || hipMalloc(&fdp, n2 * sizeof(double)) ||
+        hipMalloc(&ok, n) ||
+        (per_option &&
+         (hipMalloc(&nx, n2 * sizeof(int32_t)) || hipMalloc(&nt, n2 * sizeof(int32_t))))) {
+        cleanup();
+        return kwgpu::ctx_fail(ctx, KWGPU_ERR_NOMEM, "kwgpu_surrogate_labels: allocation");
+    }
+    GridArg g{};
+    g.x_dims = fd->x_dims;
+    g.t_dims = fd->t_dims;
+    g.x_dim = fd->x_dim;
+    g.t_dim = fd->t_dim;
+    g.density = fd->density;
+    g.scale = fd->scale;
+    // each option's two bracketing nodes, priced by the FD pricer there
+    hipLaunchKernelGGL(node_options_kernel, dim3(nblk(n, 256)), dim3(256), 0, st, d_opts, n, g,
+                       d_nodes, nx, nt, ok);
+    kwgpu_fd1d_params p = *fd;
+    p.x_dims = nx;
+    p.t_dims = nt;
+    int rc = kwgpu::fd_price_device_prec(ctx, d_nodes, n2, &p, fdp, fst, st, nullptr);
+    if (rc == KWGPU_OK) {
+        hipLaunchKernelGGL(node_premium_kernel, dim3(nblk(n, 256)), dim3(256), 0, st, d_nodes, n,
+                           fdp, fst, ok, d_premium, d_keep);
+        if (hipGetLastError() || hipStreamSynchronize(st))
+            rc = kwgpu::ctx_fail(ctx, KWGPU_ERR_HIP, "kwgpu_surrogate_labels: premium");
+    }
+    cleanup();
+    return rc;
+}
+
+int kwgpu_surrogate_fit(kwgpu_ctx* ctx, kwgpu_surrogate* s, const kwgpu_option* d_opts, size_t n,
+                        const kwgpu_fd1d_params* fd, const kwgpu_surrogate_train_params* tp,
+                        float* loss_out, size_t* n_pairs) {
+    if (n_pairs) *n_pairs = 0;
+    if (!s || !d_opts || !fd || !tp || n == 0 || tp->batch < 64 || tp->epochs < 1 ||
+        n > 0x3fffffffu || (!fd->x_dims) != (!fd->t_dims))
+        return kwgpu::ctx_fail(ctx, KWGPU_ERR_INVALID, "kwgpu_surrogate_fit: bad argument");
+    if (fd->precision != KWGPU_FP64)
+        return kwgpu::ctx_fail(ctx, KWGPU_ERR_INVALID, "kwgpu_surrogate_fit: labels are FP64 FD prices");
     HIPCHK(hipSetDevice(s->device));
     hipStream_t st = kwgpu::ctx_stream(ctx);
-    const int B = tp->batch;
-    float *feat = nullptr, *tgt = nullptr;
-    uint32_t* didx = nullptr;
-    HIPCHK(hipMalloc(&feat, n * F * sizeof(float)));
-    HIPCHK(hipMalloc(&tgt, n * sizeof(float)));
-    const int M = s->members;
-    HIPCHK(hipMalloc(&didx, (size_t)M * n * sizeof(uint32_t)));
-    int rc = KWGPU_OK;
-    do {
-        hipLaunchKernelGGL(features_kernel, dim3(nblk(n, 256)), dim3(256), 0, st, d_opts, n,
-                           d_premium, feat, tgt, (uint8_t*)nullptr);
-        if (hipError_t e = ensure_scratch(s, B)) {
-            rc = kwgpu::ctx_fail(ctx, KWGPU_ERR_NOMEM, std::string("scratch: ") + hipGetErrorString(e));
-            break;
-        }
-        // every member steps at once (one launch per layer for the whole ensemble), each
-        // on its own shuffle of the training set
-        std::vector<uint32_t> perm((size_t)M * n);
-        std::vector<std::mt19937_64> rng;
-        for (int m = 0; m < M; ++m) rng.emplace_back(tp->seed + 7919ull * (uint64_t)m);
-        const size_t steps_per_epoch = n / B;  // drop the ragged tail
-        const double total = (double)steps_per_epoch * tp->epochs;
-        const long cap = (long)s->cap;
-        hipLaunchKernelGGL(zero_f, dim3(nblk((size_t)M * kP, 256)), dim3(256), 0, st, s->m1,
-                           (size_t)M * kP);
-        hipLaunchKernelGGL(zero_f, dim3(nblk((size_t)M * kP, 256)), dim3(256), 0, st, s->m2,
-                           (size_t)M * kP);
-        long step = 0;
-        std::vector<float> lsum(M);
-        for (int ep = 0; ep < tp->epochs && rc == KWGPU_OK; ++ep) {
-            for (int m = 0; m < M; ++m) {
-                auto first = perm.begin() + (size_t)m * n;
-                std::iota(first, first + n, 0u);
-                std::shuffle(first, first + n, rng[m]);
-            }
-            if (hipError_t e = hipMemcpyAsync(didx, perm.data(), (size_t)M * n * sizeof(uint32_t),
-                                              hipMemcpyHostToDevice, st)) {
-                rc = kwgpu::ctx_fail(ctx, KWGPU_ERR_HIP,
-                                     std::string("upload permutation: ") + hipGetErrorString(e));
-                break;
-            }
-            hipLaunchKernelGGL(zero_f, dim3(1), dim3(64), 0, st, s->loss, (size_t)M);
-            for (size_t b = 0; b < steps_per_epoch; ++b, ++step) {
-                hipLaunchKernelGGL(gather_kernel, dim3(nblk(B, 256), M), dim3(256), 0, st, feat,
-                                   tgt, didx + b * B, (long)n, B, s->x, s->tg, cap);
-                hipError_t e = forward(s, 0, M, B, false, s->y, cap, true, st);
-                if (!e) e = backward(s, 0, M, B, st);
-                const double frac = step / total;
-                const float lr = (float)(tp->lr_final + 0.5 * (tp->lr - tp->lr_final) *
-                                                             (1.0 + std::cos(M_PI * frac)));
-                const float c1 = 1.0f - std::pow(tp->beta1, (float)(step + 1));
-                const float c2 = 1.0f - std::pow(tp->beta2, (float)(step + 1));
-                hipLaunchKernelGGL(adam_kernel, dim3(nblk(kP, 256), M), dim3(256), 0, st, s->pall,
-                                   s->grad, s->m1, s->m2, kP, lr, tp->beta1, tp->beta2, tp->eps,
-                                   c1, c2);
-                if (!e) e = refresh_bf16(s, 0, M, st);
-                if (e) {
-                    rc = kwgpu::ctx_fail(ctx, KWGPU_ERR_HIP,
-                                         std::string("train step: ") + hipGetErrorString(e));
-                    break;
-                }
-            }
-            // the permutation buffer is rewritten next epoch: wait before reusing it
-            if (hipError_t e = hipStreamSynchronize(st)) {
-                rc = kwgpu::ctx_fail(ctx, KWGPU_ERR_HIP, hipGetErrorString(e));
-                break;
-            }
-            (void)hipMemcpy(lsum.data(), s->loss, M * sizeof(float), hipMemcpyDeviceToHost);
-            if (loss_out)
-                for (int m = 0; m < M; ++m)
-                    loss_out[(size_t)ep * M + m] = lsum[m] / (float)(steps_per_epoch * B);
-        }
-    } while (0);
-    (void)hipStreamSynchronize(st);
-    (void)hipFree(feat);
-    (void)hipFree(tgt);
-    (void)hipFree(didx);
+    // the configuration the network will emulate
+    s->density = fd->density;
+    s->scale = fd->scale;
+    s->theta = fd->theta;
+    s->x_dim = fd->x_dim;
+    s->t_dim = fd->t_dim;
+    s->per_option = fd->x_dims != nullptr;
+    const size_t n2 = 2 * n;
+    kwgpu_option* nodes = nullptr;
+    int32_t *nx = nullptr, *nt = nullptr;
+    double* prem = nullptr;
+    uint8_t* keep = nullptr;
+    auto cleanup = [&]() {
+        (void)hipStreamSynchronize(st);
+        for (void* q : {(void*)nodes, (void*)nx, (void*)nt, (void*)prem, (void*)keep})
+            (void)hipFree(q);
+    };
+    if (hipMalloc(&nodes, n2 * sizeof(kwgpu_option)) || hipMalloc(&prem, n2 * sizeof(double)) ||
+        hipMalloc(&keep, n) ||
+        (s->per_option &&
+         (hipMalloc(&nx, n2 * sizeof(int32_t)) || hipMalloc(&nt, n2 * sizeof(int32_t))))) {
+        cleanup();
+        return kwgpu::ctx_fail(ctx, KWGPU_ERR_NOMEM, "kwgpu_surrogate_fit: allocation");
+    }
+    int rc = kwgpu_surrogate_labels(ctx, d_opts, n, fd, nodes, prem, keep);
+    std::vector<uint8_t> hkeep(n);
+    if (rc == KWGPU_OK && s->per_option) {
+        // the node rows' grids: both nodes of option i on option i's grid
+        if (hipMemcpyAsync(nx, fd->x_dims, n * sizeof(int32_t), hipMemcpyDeviceToDevice, st) ||
+            hipMemcpyAsync(nx + n, fd->x_dims, n * sizeof(int32_t), hipMemcpyDeviceToDevice, st) ||
+            hipMemcpyAsync(nt, fd->t_dims, n * sizeof(int32_t), hipMemcpyDeviceToDevice, st) ||
+            hipMemcpyAsync(nt + n, fd->t_dims, n * sizeof(int32_t), hipMemcpyDeviceToDevice, st))
+            rc = kwgpu::ctx_fail(ctx, KWGPU_ERR_HIP, "kwgpu_surrogate_fit: grids");
+    }
+    if (rc == KWGPU_OK &&
+        (hipMemcpyAsync(hkeep.data(), keep, n, hipMemcpyDeviceToHost, st) || hipStreamSynchronize(st)))
+        rc = kwgpu::ctx_fail(ctx, KWGPU_ERR_HIP, "kwgpu_surrogate_fit: labels");
+    if (rc == KWGPU_OK) {
+        std::vector<uint32_t> rows;
+        rows.reserve(n2);
+        for (size_t i = 0; i < n; ++i)
+            if (hkeep[i]) rows.push_back((uint32_t)i);
+        const size_t pairs = rows.size();
+        for (size_t k = 0; k < pairs; ++k) rows.push_back(rows[k] + (uint32_t)n);
+        if (n_pairs) *n_pairs = pairs;
+        GridArg g = grid_arg(s, nullptr);
+        g.x_dims = nx;
+        g.t_dims = nt;
+        rc = train_rows(ctx, s, nodes, prem, n2, g, &rows, tp, loss_out);
+    }
+    cleanup();
     return rc;
 }
 
 int kwgpu_surrogate_predict(kwgpu_ctx* ctx, kwgpu_surrogate* s, const kwgpu_option* d_opts,
-                            size_t n, double* d_premium, double* d_spread, uint8_t* d_in_box,
-                            void* stream) {
+                            size_t n, const kwgpu_fd1d_params* fd, double* d_premium,
+                            double* d_spread, uint8_t* d_in_box, void* stream) {
     if (!s || !d_opts || !d_premium || !d_spread)
         return kwgpu::ctx_fail(ctx, KWGPU_ERR_INVALID, "kwgpu_surrogate_predict: bad argument");
+    if (fd && !grid_matches(s, fd))
+        return kwgpu::ctx_fail(ctx, KWGPU_ERR_INVALID,
+                               "kwgpu_surrogate_predict: fd is not the configuration the "
+                               "surrogate emulates");
+    if (!fd && s->per_option)
+        return kwgpu::ctx_fail(ctx, KWGPU_ERR_INVALID,
+                               "kwgpu_surrogate_predict: per-option grids needed");
     if (n == 0) return KWGPU_OK;
     hipStream_t st = stream ? (hipStream_t)stream : kwgpu::ctx_stream(ctx);
-    constexpr size_t CH = 32768;  // samples per pass (scratch: ~1 GB for 4 members)
-    HIPCHK(ensure_scratch(s, std::min(n, CH)));
-    const size_t need = (size_t)s->members * std::min(n, CH);
+    if (s->done_stream && s->done_stream != st) HIPCHK(hipStreamWaitEvent(st, s->ev_done, 0));
+    // two rows per option (the bracketing nodes); scratch for 2 CH rows per member
+    constexpr size_t CH = 16384;
+    HIPCHK(ensure_scratch(s, 2 * std::min(n, CH)));
+    const size_t need = (size_t)s->members * 2 * std::min(n, CH);
     if (s->yall_cap < need) {
         if (s->yall) HIPCHK(hipFree(s->yall));
         s->yall = nullptr;
         HIPCHK(hipMalloc(&s->yall, need * sizeof(float)));
         s->yall_cap = need;
     }
+    const GridArg g0 = grid_arg(s, fd);
     for (size_t c0 = 0; c0 < n; c0 += CH) {
-        const int cn = (int)std::min(CH, n - c0);
-        hipLaunchKernelGGL(features_kernel, dim3(nblk(cn, 256)), dim3(256), 0, st, d_opts + c0,
-                           (size_t)cn, (const double*)nullptr, s->x, (float*)nullptr,
-                           d_in_box ? d_in_box + c0 : nullptr);
+        const size_t cn = std::min(CH, n - c0);
+        GridArg g = g0;
+        if (g.x_dims) g.x_dims += c0;
+        if (g.t_dims) g.t_dims += c0;
+        hipLaunchKernelGGL(node_features_kernel, dim3(nblk(cn, 256)), dim3(256), 0, st,
+                           d_opts + c0, cn, g, s->x, d_in_box ? d_in_box + c0 : nullptr);
         // the whole ensemble at once on the shared features
-        HIPCHK(forward(s, 0, s->members, cn, true, s->yall, cn, false, st));
-        hipLaunchKernelGGL(combine_kernel, dim3(nblk(cn, 256)), dim3(256), 0, st, d_opts + c0,
-                           s->yall, s->members, (size_t)cn, (size_t)cn, d_premium + c0,
+        HIPCHK(forward(s, 0, s->members, (int)(2 * cn), true, s->yall, (long)(2 * cn), false, st));
+        hipLaunchKernelGGL(node_combine_kernel, dim3(nblk(cn, 256)), dim3(256), 0, st,
+                           d_opts + c0, cn, g, s->yall, s->members, 2 * cn, d_premium + c0,
                            d_spread + c0);
         HIPCHK(hipGetLastError());
     }
+    HIPCHK(hipEventRecord(s->ev_done, st));
+    s->done_stream = st;
     return KWGPU_OK;
 }
 
@@ -1206,10 +1541,11 @@ int kwgpu_surrogate_price(kwgpu_ctx* ctx, kwgpu_surrogate* s, const kwgpu_option
     int* cnt = (int*)(base + at);
     int rc = KWGPU_OK;
     auto cleanup = [&]() { (void)hipStreamSynchronize(st); };
-    // a negative (or NaN) bound passes no option: the ensemble is not evaluated
-    const bool serve_none = !(spread_max >= 0.0);
+    // a negative (or NaN) bound passes no option, and neither does an FD configuration
+    // other than the one the surrogate emulates: the ensemble is not evaluated
+    const bool serve_none = !(spread_max >= 0.0) || !grid_matches(s, fd);
     if (!serve_none) {
-        rc = kwgpu_surrogate_predict(ctx, s, d_opts, n, prem, spr, inb, st);
+        rc = kwgpu_surrogate_predict(ctx, s, d_opts, n, fd, prem, spr, inb, st);
         if (rc) {
             cleanup();
             return rc;
@@ -1274,6 +1610,11 @@ int kwgpu_surrogate_save(kwgpu_ctx* ctx, const kwgpu_surrogate* s, const char* p
     const uint32_t hdr[4] = {0x4b575355u /* "KWSU" */, kFileVersion, (uint32_t)s->members,
                              (uint32_t)kP};
     std::fwrite(hdr, sizeof(hdr), 1, f);
+    // the FD configuration it emulates: density, scale, theta, x and t points, per option
+    const double gd[3] = {s->density, s->scale, s->theta};
+    const int32_t gi[3] = {s->x_dim, s->t_dim, s->per_option ? 1 : 0};
+    std::fwrite(gd, sizeof(gd), 1, f);
+    std::fwrite(gi, sizeof(gi), 1, f);
     std::vector<float> w(kP);
     for (int m = 0; m < s->members; ++m) {
         if (hipMemcpy(w.data(), s->p[m], kP * sizeof(float), hipMemcpyDeviceToHost)) {
@@ -1296,7 +1637,22 @@ int kwgpu_surrogate_load(kwgpu_ctx* ctx, const char* path, kwgpu_surrogate** out
         std::fclose(f);
         return kwgpu::ctx_fail(ctx, KWGPU_ERR_INVALID, "kwgpu_surrogate_load: not a surrogate file");
     }
+    double gd[3] = {};
+    int32_t gi[3] = {};
+    if (std::fread(gd, sizeof(gd), 1, f) != 1 || std::fread(gi, sizeof(gi), 1, f) != 1 ||
+        gi[0] < 3 || gi[1] < 2) {
+        std::fclose(f);
+        return kwgpu::ctx_fail(ctx, KWGPU_ERR_INVALID, "kwgpu_surrogate_load: truncated file");
+    }
     int rc = kwgpu_surrogate_create(ctx, (int32_t)hdr[2], 0, out);
+    if (rc == KWGPU_OK) {
+        (*out)->density = gd[0];
+        (*out)->scale = gd[1];
+        (*out)->theta = gd[2];
+        (*out)->x_dim = gi[0];
+        (*out)->t_dim = gi[1];
+        (*out)->per_option = gi[2] != 0;
+    }
     std::vector<float> w(kP);
     for (uint32_t m = 0; rc == KWGPU_OK && m < hdr[2]; ++m) {
         if (std::fread(w.data(), sizeof(float), kP, f) != (size_t)kP)

@@ -24,15 +24,18 @@ LAYOUT_AUTO, LAYOUT_LANE_SOA, LAYOUT_WAVE_REG = 0, 1, 2
 
 EXPORTED_SYMBOLS = (
     "kwgpu_fd1d_params_default", "kwgpu_create", "kwgpu_destroy", "kwgpu_last_error",
-    "kwgpu_fd1d_price", "kwgpu_fd1d_price_device", "kwgpu_bs_price", "kwgpu_bs_price_device",
+    "kwgpu_fd1d_price", "kwgpu_fd1d_price_device",
+    "kwgpu_bs_price", "kwgpu_bs_price_device",
     "kwgpu_fd1d_bs_price", "kwgpu_fd1d_greeks", "kwgpu_synchronize", "kwgpu_stream",
     "kwgpu_surrogate_train_params_default", "kwgpu_surrogate_create", "kwgpu_surrogate_destroy",
-    "kwgpu_surrogate_train", "kwgpu_surrogate_predict", "kwgpu_surrogate_price",
+    "kwgpu_surrogate_train", "kwgpu_surrogate_labels", "kwgpu_surrogate_fit",
+    "kwgpu_surrogate_predict",
+    "kwgpu_surrogate_price",
     "kwgpu_surrogate_get_params", "kwgpu_surrogate_set_params", "kwgpu_surrogate_loss_grad",
     "kwgpu_surrogate_features", "kwgpu_surrogate_baseline", "kwgpu_surrogate_save", "kwgpu_surrogate_load",
     "kwgpu_fd1d_autotune", "kwgpu_tune_save", "kwgpu_tune_load",
 )
-SURROGATE_PARAMS = 199425
+SURROGATE_PARAMS = 199937
 GREEKS_DTYPE = np.dtype([(f, "<f8") for f in ("price", "delta", "gamma", "vega", "theta", "rho")])
 
 
@@ -92,14 +95,18 @@ def lib():
         L.kwgpu_surrogate_destroy.argtypes = [P]
         L.kwgpu_surrogate_destroy.restype = None
         L.kwgpu_surrogate_train.argtypes = [P, P, P, P, SZ, ctypes.POINTER(SurrogateTrainParams), P]
-        L.kwgpu_surrogate_predict.argtypes = [P, P, P, SZ, P, P, P, P]
+        L.kwgpu_surrogate_labels.argtypes = [P, P, SZ, ctypes.POINTER(Fd1dParams), P, P, P]
+        L.kwgpu_surrogate_fit.argtypes = [P, P, P, SZ, ctypes.POINTER(Fd1dParams),
+                                          ctypes.POINTER(SurrogateTrainParams), P,
+                                          ctypes.POINTER(ctypes.c_size_t)]
+        L.kwgpu_surrogate_predict.argtypes = [P, P, P, SZ, ctypes.POINTER(Fd1dParams), P, P, P, P]
         L.kwgpu_surrogate_price.argtypes = [P, P, P, SZ, ctypes.c_double,
                                             ctypes.POINTER(Fd1dParams), P, P,
                                             ctypes.POINTER(ctypes.c_size_t)]
         L.kwgpu_surrogate_get_params.argtypes = [P, P, ctypes.c_int32, P]
         L.kwgpu_surrogate_set_params.argtypes = [P, P, ctypes.c_int32, P]
         L.kwgpu_surrogate_loss_grad.argtypes = [P, P, ctypes.c_int32, P, P, SZ, P, P]
-        L.kwgpu_surrogate_features.argtypes = [P, P, SZ, P]
+        L.kwgpu_surrogate_features.argtypes = [P, P, SZ, ctypes.c_int32, ctypes.c_int32, P]
         L.kwgpu_surrogate_baseline.argtypes = [P, P, SZ, P]
         L.kwgpu_surrogate_save.argtypes = [P, P, ctypes.c_char_p]
         L.kwgpu_surrogate_load.argtypes = [P, ctypes.c_char_p, ctypes.POINTER(P)]
@@ -310,20 +317,41 @@ class Surrogate:
     def save(self, path: str):
         self._check(lib().kwgpu_surrogate_save(self.ctx.h, self.h, path.encode()))
 
-    def train(self, d_opts: int, d_premium: int, n: int, epochs=30, batch=8192, lr=2e-3,
-              lr_final=2e-5, seed=7) -> np.ndarray:
+    @staticmethod
+    def _train_params(epochs, batch, lr, lr_final, seed):
         tp = SurrogateTrainParams()
         lib().kwgpu_surrogate_train_params_default(ctypes.byref(tp))
         tp.epochs, tp.batch, tp.lr, tp.lr_final, tp.seed = epochs, batch, lr, lr_final, seed
+        return tp
+
+    def train(self, d_opts: int, d_premium: int, n: int, epochs=30, batch=8192, lr=2e-3,
+              lr_final=2e-5, seed=7) -> np.ndarray:
+        """Trains on (option, premium) rows on the surrogate's single grid."""
+        tp = self._train_params(epochs, batch, lr, lr_final, seed)
         loss = np.zeros((epochs, self.members), np.float32)
         self._check(lib().kwgpu_surrogate_train(self.ctx.h, self.h, d_opts, d_premium, n,
                                                 ctypes.byref(tp), loss.ctypes.data))
         return loss
 
+    def fit(self, d_opts: int, n: int, fd: Fd1dParams, epochs=30, batch=8192, lr=2e-3,
+            lr_final=2e-5, seed=7):
+        """Labels each option's two bracketing mesh nodes with the FD pricer configured by
+        `fd` and trains on them; returns (losses[epochs, members], options kept)."""
+        tp = self._train_params(epochs, batch, lr, lr_final, seed)
+        loss = np.zeros((epochs, self.members), np.float32)
+        kept = ctypes.c_size_t(0)
+        self._check(lib().kwgpu_surrogate_fit(self.ctx.h, self.h, d_opts, n, ctypes.byref(fd),
+                                              ctypes.byref(tp), loss.ctypes.data,
+                                              ctypes.byref(kept)))
+        return loss, kept.value
+
     def predict(self, d_opts: int, n: int, d_premium: int, d_spread: int, d_in_box: int,
-                stream: int | None = None):
-        self._check(lib().kwgpu_surrogate_predict(self.ctx.h, self.h, d_opts, n, d_premium,
-                                                  d_spread, d_in_box, stream))
+                fd: Fd1dParams | None = None, stream: int | None = None):
+        """Premium (surrogate price - BS at the spot), spread and in-box flags; `fd` is the
+        emulated configuration (needed for per-option grids)."""
+        self._check(lib().kwgpu_surrogate_predict(self.ctx.h, self.h, d_opts, n,
+                                                  None if fd is None else ctypes.byref(fd),
+                                                  d_premium, d_spread, d_in_box, stream))
 
     def price(self, d_opts: int, n: int, spread_max: float, fd: Fd1dParams, d_prices: int,
               d_route: int) -> int:
@@ -358,14 +386,29 @@ class Surrogate:
         return float(loss.value), g
 
 
-def surrogate_features(opts) -> np.ndarray:
-    """Normalised network inputs (n x 6) of host options, as the kernels compute them."""
+SURROGATE_FEATURES = 8
+
+
+def surrogate_features(opts, x_dim=1024, t_dim=1024) -> np.ndarray:
+    """Normalised network inputs (n x 8) of host options priced on an x_dim x t_dim grid,
+    as the kernels compute them."""
     a = as_options(opts)
-    out = np.zeros((len(a), 6), np.float32)
-    rc = lib().kwgpu_surrogate_features(None, a.ctypes.data, len(a), out.ctypes.data)
+    out = np.zeros((len(a), SURROGATE_FEATURES), np.float32)
+    rc = lib().kwgpu_surrogate_features(None, a.ctypes.data, len(a), x_dim, t_dim,
+                                        out.ctypes.data)
     if rc != OK:
         raise KwgpuError(rc, "kwgpu_surrogate_features failed")
     return out
+
+
+def surrogate_labels(ctx: Context, d_opts: int, n: int, fd: Fd1dParams, d_nodes: int,
+                     d_premium: int, d_keep: int):
+    """kwgpu_surrogate_labels: node options (2n), their FD premiums (2n) and keep flags
+    (n), device pointers."""
+    rc = lib().kwgpu_surrogate_labels(ctx.h, d_opts, n, ctypes.byref(fd), d_nodes, d_premium,
+                                      d_keep)
+    if rc != OK:
+        raise KwgpuError(rc, ctx.last_error)
 
 
 def surrogate_baseline(ctx: Context, opts) -> np.ndarray:

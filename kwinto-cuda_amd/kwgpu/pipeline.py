@@ -1,16 +1,19 @@
-"""Surrogate pipeline: labels from the GPU FD pricer, ensemble training, held-out
-evaluation against FD, gate calibration and surrogate-first serving.
+"""Surrogate pipeline: fitting to the GPU FD pricer, held-out evaluation against FD,
+gate calibration and surrogate-first serving.
 
 Everything runs on the device through the kwgpu C-ABI; torch provides device memory
 and timing events only.
 
-Workloads (kwgpu.workloads): the training set is `surrogate_train` (K U[40,200],
-T U[1/52,5], the plain-method sigma/r/q/w draws, seed 3); the serving batch is
-`serving` (the same box with ~10% of options moved outside it, seed 4). Labels use one
-grid, 1024 x-points x 1024 t-points. Serving uses that grid for the FD fallback too, so
-served prices and the FP64 FD prices they are compared with describe the same
-discretisation, or per-option grids (`serve(..., x_dims, t_dims)`), against which FP64
-FD on those same grids is the reference.
+A surrogate emulates one FD configuration. `train_and_evaluate` fits it to the FD pricer
+on one grid (1024 x-points x 1024 t-points) or, with `per_option_grids`, on per-option
+grids drawn by the serving batch's rule (x-points from {256, 512, 1024, 2048}, t-points
+max(32, ceil(365 T))). Labels are the FD pricer's values at the two mesh nodes that
+bracket each training option's log(S/K) (kwgpu_surrogate_fit); the held-out check prices
+options at their own spots, where the surrogate interpolates between its node values
+exactly as the FD pricer interpolates between its own, against the FD price on the same
+grid. Workloads (kwgpu.workloads): the training set is `surrogate_train` (K U[40,200],
+T U[1/52,5], the plain-method sigma/r/q/w draws, seed 3); the serving batch is `serving`
+(the same box with ~10% of options moved outside it, seed 4).
 """
 from __future__ import annotations
 
@@ -67,36 +70,31 @@ def bs_euro(ctx: Context, opts) -> "object":
     return d_b
 
 
-def premium_labels(ctx: Context, opts, chunk=1 << 18):
-    """(device options, device premium = FD FP64 - BS, kept) for a host option array.
-    Options the FD pricer rejects (a spot outside its grid: the box's corner K = 200,
-    sigma = 0.1, T = 1/52 has log(S/K) at the grid's end) are dropped; `kept` is the
-    host index of the options returned."""
+def device_dims(n, x_dims=None, t_dims=None):
+    """Device int32 tensors of per-option grid sizes (or (None, None))."""
     torch = _torch()
-    n = len(opts)
-    d_o = to_device(opts)
-    prem = torch.empty(n, dtype=torch.float64, device="cuda")
-    st = torch.empty(n, dtype=torch.int32, device="cuda")
-    bs = bs_euro(ctx, opts)
-    for c0 in range(0, n, chunk):
-        c1 = min(n, c0 + chunk)
-        prem[c0:c1], st[c0:c1] = fd_prices(ctx, d_o[c0 * 56:c1 * 56], c1 - c0, with_status=True)
-    prem -= bs
-    ok = (st == 0).cpu().numpy()
-    kept = np.nonzero(ok)[0]
-    if len(kept) < n:
-        sel = torch.from_numpy(kept).cuda()
-        d_o = d_o.view(n, 56)[sel].reshape(-1).contiguous()
-        prem = prem[sel].contiguous()
-    return d_o, prem, kept
+    if x_dims is None and t_dims is None:
+        return None, None
+    xd = np.full(n, X_DIM, np.int32) if x_dims is None else np.asarray(x_dims, np.int32)
+    td = np.full(n, T_DIM, np.int32) if t_dims is None else np.asarray(t_dims, np.int32)
+    assert xd.shape == (n,) and td.shape == (n,)
+    return torch.from_numpy(xd.copy()).cuda(), torch.from_numpy(td.copy()).cuda()
 
 
-def predict(ctx: Context, sur: Surrogate, d_opts, n: int):
+def fd_config(d_x=None, d_t=None, precision=FP64):
+    """The FD configuration: 1024 x 1024, or per-option grids from device tensors."""
+    dims = {}
+    if d_x is not None:
+        dims = {"x_dims": d_x.data_ptr(), "t_dims": d_t.data_ptr()}
+    return params(X_DIM, T_DIM, precision=precision, **dims)
+
+
+def predict(ctx: Context, sur: Surrogate, d_opts, n: int, fd=None):
     torch = _torch()
     d_pm = torch.empty(n, dtype=torch.float64, device="cuda")
     d_sp = torch.empty(n, dtype=torch.float64, device="cuda")
     d_ib = torch.empty(n, dtype=torch.uint8, device="cuda")
-    sur.predict(d_opts.data_ptr(), n, d_pm.data_ptr(), d_sp.data_ptr(), d_ib.data_ptr())
+    sur.predict(d_opts.data_ptr(), n, d_pm.data_ptr(), d_sp.data_ptr(), d_ib.data_ptr(), fd)
     ctx.synchronize()
     return d_pm, d_sp, d_ib
 
@@ -121,49 +119,59 @@ def calibrate_spread(err, spread, bar=ERR_BAR, cap=SPREAD_MAX, margin=0.5,
     return float(s[idx[-1]])
 
 
-def train_and_evaluate(ctx: Context, n_train=2_000_000, n_test=65536, members=4, epochs=30,
-                       batch=8192, seed=3, log=print):
-    """Trains the ensemble on FD labels and evaluates it on held-out FD labels."""
-    t0 = time.perf_counter()
+def train_and_evaluate(ctx: Context, n_train=2_000_000, n_test=65536, members=4, epochs=160,
+                       batch=8192, seed=3, per_option_grids=False, log=print):
+    """Fits the ensemble to the FD pricer and evaluates it on held-out options against FD
+    prices at their spots (same configuration)."""
+    torch = _torch()
     opts = W.surrogate_train(n_train + n_test, seed)
-    d_o, prem, kept = premium_labels(ctx, opts)
-    t_label = time.perf_counter() - t0
-    dropped = n_train + n_test - len(kept)
-    n_train = int(np.sum(kept < n_train))  # kept options keep their order
-    n_test = len(kept) - n_train
-    log(f"labels: {len(kept)} options by FD FP64 {X_DIM}x{T_DIM} in {t_label:.2f} s "
-        f"({dropped} outside their grid dropped)")
+    dims = W.serving_grids(opts, seed + 100) if per_option_grids else (None, None)
+    d_x, d_t = device_dims(len(opts), *dims)
+    d_o = to_device(opts)
     sur = Surrogate(ctx, members=members, seed=seed)
+    fd_tr = fd_config(None if d_x is None else d_x[:n_train],
+                      None if d_t is None else d_t[:n_train])
     t0 = time.perf_counter()
-    loss = sur.train(d_o.data_ptr(), prem.data_ptr(), n_train, epochs=epochs, batch=batch)
-    t_train = time.perf_counter() - t0
-    log(f"training: {members} members x {epochs} epochs in {t_train:.2f} s, "
+    loss, pairs = sur.fit(d_o.data_ptr(), n_train, fd_tr, epochs=epochs, batch=batch)
+    t_fit = time.perf_counter() - t0
+    grid = "per-option grids" if per_option_grids else f"{X_DIM}x{T_DIM}"
+    log(f"fit: {pairs} of {n_train} options labelled at their two bracketing nodes "
+        f"(FD FP64, {grid}), {members} members x {epochs} epochs in {t_fit:.2f} s, "
         f"final loss {loss[-1].tolist()}")
+    # held-out options at their own spots, against the FD price on the same grid
     d_te = d_o[n_train * 56:]
-    pm, sp, ib = predict(ctx, sur, d_te, n_test)
-    truth = prem[n_train:].cpu().numpy()
-    err = pm.cpu().numpy() - truth
+    fd_te = fd_config(None if d_x is None else d_x[n_train:], None if d_t is None else d_t[n_train:])
+    truth_d = torch.empty(n_test, dtype=torch.float64, device="cuda")
+    st_d = torch.empty(n_test, dtype=torch.int32, device="cuda")
+    ctx.fd1d_price_device(d_te.data_ptr(), n_test, truth_d.data_ptr(), st_d.data_ptr(), fd_te)
+    pm, sp, ib = predict(ctx, sur, d_te, n_test, fd_te)
+    te = opts[n_train:]
+    bs = bs_euro(ctx, te).cpu().numpy()
+    truth, st = truth_d.cpu().numpy(), st_d.cpu().numpy()
+    inb = (ib.cpu().numpy() == 1) & (st == 0)
+    err = (bs + pm.cpu().numpy()) - truth
     spread = sp.cpu().numpy()
-    thr = calibrate_spread(err, spread)
-    served = spread <= thr
+    e_in, s_in = err[inb], spread[inb]
+    thr = calibrate_spread(e_in, s_in)
+    served = s_in <= thr
     # the same calibration at looser error bars, for the accuracy / throughput trade-off
     looser = {}
     for bar in (1e-3, 1e-2):
-        tb = calibrate_spread(err, spread, bar=bar, cap=bar / 2)
-        sv = spread <= tb
+        tb = calibrate_spread(e_in, s_in, bar=bar, cap=bar / 2)
+        sv = s_in <= tb
         looser[f"{bar:g}"] = {"spread_bound": tb, "heldout_pass_fraction": float(np.mean(sv)),
                               "heldout_max_err_served":
-                                  float(np.max(np.abs(err[sv]))) if sv.any() else 0.0}
+                                  float(np.max(np.abs(e_in[sv]))) if sv.any() else 0.0}
     out = {
-        "n_train": n_train, "n_test": n_test, "dropped_outside_grid": dropped,
-        "members": members, "epochs": epochs,
-        "label_s": t_label, "train_s": t_train, "final_loss": loss[-1].tolist(),
-        "heldout_max_abs_err": float(np.max(np.abs(err))),
-        "heldout_rms_abs_err": float(np.sqrt(np.mean(err * err))),
-        "label_rms": float(np.sqrt(np.mean((truth - truth.mean()) ** 2))),
+        "grid": grid, "n_train": n_train, "n_labelled_pairs": int(pairs), "n_test": n_test,
+        "n_test_in_box": int(inb.sum()), "members": members, "epochs": epochs,
+        "fit_s": t_fit, "final_loss": loss[-1].tolist(),
+        "heldout_max_abs_err": float(np.max(np.abs(e_in))),
+        "heldout_rms_abs_err": float(np.sqrt(np.mean(e_in * e_in))),
+        "heldout_frac_within_bar": float(np.mean(np.abs(e_in) <= ERR_BAR)),
         "spread_bound": thr,
         "heldout_pass_fraction": float(np.mean(served)),
-        "heldout_max_err_served": float(np.max(np.abs(err[served]))) if served.any() else 0.0,
+        "heldout_max_err_served": float(np.max(np.abs(e_in[served]))) if served.any() else 0.0,
         "looser_bars": looser,
     }
     return sur, out
@@ -181,16 +189,9 @@ def serve(ctx: Context, sur: Surrogate, opts, spread_bound: float, repeats=3, lo
     d_o = to_device(opts)
     d_p = torch.empty(n, dtype=torch.float64, device="cuda")
     d_r = torch.empty(n, dtype=torch.int32, device="cuda")
-    dims = {}
-    if x_dims is not None or t_dims is not None:
-        xd = np.full(n, X_DIM, np.int32) if x_dims is None else np.asarray(x_dims, np.int32)
-        td = np.full(n, T_DIM, np.int32) if t_dims is None else np.asarray(t_dims, np.int32)
-        assert xd.shape == (n,) and td.shape == (n,)
-        d_x = torch.from_numpy(xd.copy()).cuda()
-        d_t = torch.from_numpy(td.copy()).cuda()
-        dims = {"x_dims": d_x.data_ptr(), "t_dims": d_t.data_ptr()}
-    fd = params(X_DIM, T_DIM, precision=precision, **dims)
-    fd64 = params(X_DIM, T_DIM, precision=FP64, **dims)
+    d_x, d_t = device_dims(n, x_dims, t_dims)
+    fd = fd_config(d_x, d_t, precision)
+    fd64 = fd_config(d_x, d_t, FP64)
     d_ref = torch.empty(n, dtype=torch.float64, device="cuda")
     d_st = torch.empty(n, dtype=torch.int32, device="cuda")
 

@@ -240,29 +240,62 @@ int main(int argc, char** argv) {
                c, "surrogate train");
         std::vector<double> pm(n), sp(n);
         std::vector<uint8_t> ib(n);
-        expect(kwgpu_surrogate_predict(c, s, o.data(), n, pm.data(), sp.data(), ib.data(), nullptr),
+        expect(kwgpu_surrogate_predict(c, s, o.data(), n, nullptr, pm.data(), sp.data(), ib.data(),
+                                       nullptr),
                KWGPU_OK, c, "surrogate predict");
+        // fitted to the FD pricer: node labels on one grid, then on per-option grids. No
+        // kernel runs here, so no node is labelled (the node grids read back as zeros) and
+        // the fit stops before training with KWGPU_ERR_INVALID; what is checked is the
+        // path up to there, its buffers and the configuration it records.
         Sizes g = sizes(n, {64, 128, 256, 512}, 20, 40, 6);
+        size_t pairs = 0;
+        kwgpu_fd1d_params_default(&p);
+        p.x_dim = 128;
+        p.t_dim = 32;
+        expect(kwgpu_surrogate_fit(c, s, o.data(), n, &p, &tp, loss.data(), &pairs),
+               KWGPU_ERR_INVALID, c, "surrogate fit");
+        expect(kwgpu_surrogate_predict(c, s, o.data(), n, &p, pm.data(), sp.data(), ib.data(),
+                                       nullptr),
+               KWGPU_OK, c, "predict on the fitted grid");
+        kwgpu_fd1d_params other = p;
+        other.x_dim = 256;
+        expect(kwgpu_surrogate_predict(c, s, o.data(), n, &other, pm.data(), sp.data(), ib.data(),
+                                       nullptr),
+               KWGPU_ERR_INVALID, c, "predict on another grid");
+        kwgpu_surrogate* sg = nullptr;
+        expect(kwgpu_surrogate_create(c, 2, 6, &sg), KWGPU_OK, c, "second surrogate");
+        p.x_dims = g.x.data();
+        p.t_dims = g.t.data();
+        expect(kwgpu_surrogate_fit(c, sg, o.data(), n, &p, &tp, loss.data(), &pairs),
+               KWGPU_ERR_INVALID, c, "surrogate fit, per-option grids");
+        expect(kwgpu_surrogate_predict(c, sg, o.data(), n, nullptr, pm.data(), sp.data(), ib.data(),
+                                       nullptr),
+               KWGPU_ERR_INVALID, c, "per-option surrogate without grids");
+        expect(kwgpu_surrogate_predict(c, sg, o.data(), n, &p, pm.data(), sp.data(), ib.data(),
+                                       nullptr),
+               KWGPU_OK, c, "per-option surrogate predict");
         std::vector<double> pr(n);
         std::vector<int32_t> route(n);
         size_t nfd = 0;
-        for (int prec : {KWGPU_FP64, KWGPU_FP32_MIXED})
-            for (double bound : {-1.0, 0.0, 1e300})
-                for (bool per_option : {false, true}) {
-                    kwgpu_fd1d_params_default(&p);
-                    p.x_dim = 128;
-                    p.t_dim = 32;
-                    p.precision = prec;
-                    if (per_option) {
-                        p.x_dims = g.x.data();
-                        p.t_dims = g.t.data();
+        for (kwgpu_surrogate* sx : {s, sg})
+            for (int prec : {KWGPU_FP64, KWGPU_FP32_MIXED})
+                for (double bound : {-1.0, 0.0, 1e300})
+                    for (bool per_option : {false, true}) {
+                        kwgpu_fd1d_params_default(&p);
+                        p.x_dim = 128;
+                        p.t_dim = 32;
+                        p.precision = prec;
+                        if (per_option) {
+                            p.x_dims = g.x.data();
+                            p.t_dims = g.t.data();
+                        }
+                        // a smaller batch after a larger one reuses the serving buffers
+                        for (size_t m : {n, n / 3})
+                            expect(kwgpu_surrogate_price(c, sx, o.data(), m, bound, &p, pr.data(),
+                                                         route.data(), &nfd),
+                                   KWGPU_OK, c, "surrogate price");
                     }
-                    // a smaller batch after a larger one reuses the serving buffers
-                    for (size_t m : {n, n / 3})
-                        expect(kwgpu_surrogate_price(c, s, o.data(), m, bound, &p, pr.data(),
-                                                     route.data(), &nfd),
-                               KWGPU_OK, c, "surrogate price");
-                }
+        kwgpu_surrogate_destroy(sg);
         const std::string path = dir + "/sur.bin";
         expect(kwgpu_surrogate_save(c, s, path.c_str()), KWGPU_OK, c, "surrogate save");
         kwgpu_surrogate* s2 = nullptr;

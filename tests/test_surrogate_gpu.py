@@ -17,8 +17,8 @@ from kwgpu import workloads as W
 
 pytestmark = pytest.mark.gpu
 
-H, F = 256, 6
-kW1, kB1 = 0, 256 * 6
+H, F = 256, 8
+kW1, kB1 = 0, 256 * 8
 kW2 = kB1 + H
 kB2 = kW2 + H * H
 kW3 = kB2 + H
@@ -177,34 +177,95 @@ def _fd_premium(ctx, opts, x_dim, t_dim, precision=kwgpu.FP64):
     return d_o, d_p - d_b
 
 
-def test_training_learns_the_premium(ctx):
-    """A short training run on FD labels (256 x 128 grid) lowers the loss and predicts
-    held-out premiums far better than their mean."""
+def _device(opts):
     import torch
+    return torch.from_numpy(np.ascontiguousarray(opts).view(np.uint8).copy()).cuda()
+
+
+def _predict(s, d_o, n, fd):
+    import torch
+    d_pm = torch.empty(n, dtype=torch.float64, device="cuda")
+    d_sp = torch.empty(n, dtype=torch.float64, device="cuda")
+    d_ib = torch.empty(n, dtype=torch.uint8, device="cuda")
+    s.predict(d_o.data_ptr(), n, d_pm.data_ptr(), d_sp.data_ptr(), d_ib.data_ptr(), fd)
+    s.ctx.synchronize()
+    return d_pm.cpu().numpy(), d_sp.cpu().numpy(), d_ib.cpu().numpy()
+
+
+def test_training_learns_the_premium(ctx):
+    """A short fit to the FD pricer on a 256 x 128 grid (labels at the two nodes
+    bracketing each option's log(S/K)) lowers the loss, and the surrogate prices held-out
+    options at their own spots far better than the premiums' spread."""
     _say("training")
     o = W.surrogate_train(40000, 31)
-    d_o, prem = _fd_premium(ctx, o, 256, 128)
+    fd = kwgpu.params(256, 128)
     n_tr = 32768
+    d_o = _device(o)
     s = kwgpu.Surrogate(ctx, members=2, seed=3)
-    loss = s.train(d_o.data_ptr(), prem.data_ptr(), n_tr, epochs=12, batch=2048, lr=2e-3)
-    _say("  loss per epoch", loss[:, 0])
+    loss, pairs = s.fit(d_o.data_ptr(), n_tr, fd, epochs=12, batch=2048, lr=2e-3)
+    _say("  loss per epoch", loss[:, 0], "pairs", pairs)
+    assert pairs >= n_tr - 16  # the box's far corner can put a spot in the first cell
     assert np.all(np.isfinite(loss))
     assert loss[-1].max() < loss[0].min() / 4
-    n_te = len(o) - n_tr
-    d_te = d_o[n_tr * 56:]
-    d_pm = torch.empty(n_te, dtype=torch.float64, device="cuda")
-    d_sp = torch.empty(n_te, dtype=torch.float64, device="cuda")
-    d_ib = torch.empty(n_te, dtype=torch.uint8, device="cuda")
-    s.predict(d_te.data_ptr(), n_te, d_pm.data_ptr(), d_sp.data_ptr(), d_ib.data_ptr())
-    ctx.synchronize()
-    truth = prem[n_tr:].cpu().numpy()
-    pred = d_pm.cpu().numpy()
-    assert np.all(d_ib.cpu().numpy() == 1)
-    rms = np.sqrt(np.mean((pred - truth) ** 2))
-    base = np.sqrt(np.mean((truth - truth.mean()) ** 2))
-    _say(f"  held-out rms {rms:.3e} vs spread of the labels {base:.3e}")
+    te = o[n_tr:]
+    n_te = len(te)
+    pm, sp, ib = _predict(s, d_o[n_tr * 56:], n_te, fd)
+    fdp, st, rc, _ = ctx.fd1d_price(te, 256, 128)
+    e = te.copy()
+    e["e"] = 0
+    bs = ctx.bs_price(e)
+    ok = (ib == 1) & (st == 0)
+    assert ok.mean() > 0.99
+    truth = fdp - bs
+    rms = np.sqrt(np.mean((pm[ok] - truth[ok]) ** 2))
+    base = np.sqrt(np.mean((truth[ok] - truth[ok].mean()) ** 2))
+    _say(f"  held-out rms {rms:.3e} vs spread of the premiums {base:.3e}")
     assert rms < base / 10
-    assert np.all(d_sp.cpu().numpy() >= 0)
+    assert np.all(sp[ok] >= 0)
+    # another configuration is refused
+    with pytest.raises(kwgpu.KwgpuError):
+        _predict(s, d_o[n_tr * 56:], n_te, kwgpu.params(512, 128))
+    s.close()
+
+
+def test_node_form_matches_fd_interpolation(ctx):
+    """The surrogate forms a price from its two node values as the FD pricer does
+    (src/Math/kwFd1d.cpp:139-158). With the network's weights zeroed except the output
+    bias, the premium at every node is baw + (baw + 1e-4 K) b5, so the price is BS plus
+    that, interpolated with the reference's weights at the bracketing nodes (computed
+    here from the sinh mesh) -- to rounding."""
+    from baw_ref import baw_premium
+    o = W.surrogate_train(2000, 77)
+    fd = kwgpu.params(512, 256)
+    s = kwgpu.Surrogate(ctx, members=1, seed=1)
+    d_o = _device(o)
+    s.fit(d_o.data_ptr(), 2000, fd, epochs=1, batch=1024)  # records the configuration
+    w = np.zeros(kwgpu.SURROGATE_PARAMS, np.float32)
+    b5 = 0.37
+    w[-1] = b5
+    s.set_params(0, w)
+    pm, sp, ib = _predict(s, d_o, len(o), fd)
+    ok = ib == 1
+    assert ok.mean() > 0.99 and np.all(sp[ok] == 0)
+    # bracketing nodes on the 512-point sinh mesh (first node >= log(S/K))
+    xs = np.log(o["s"] / o["k"])
+    xmax = 50.0 * o["z"] * np.sqrt(o["t"])
+    ymin, ymax = np.arcsinh(-xmax / 0.25), np.arcsinh(xmax / 0.25)
+    j = np.arange(512)[None, :] / 511.0
+    nodes = 0.25 * np.sinh(ymin[:, None] * (1 - j) + ymax[:, None] * j)
+    jj = np.argmax(nodes >= xs[:, None], axis=1)
+    x0, x1 = nodes[np.arange(len(o)), jj - 1], nodes[np.arange(len(o)), jj]
+    a, b = o.copy(), o.copy()
+    a["s"], b["s"] = o["k"] * np.exp(x0), o["k"] * np.exp(x1)
+    ea, eb, es = a.copy(), b.copy(), o.copy()
+    for q in (ea, eb, es):
+        q["e"] = 0
+    va = ctx.bs_price(ea) + baw_premium(a) + (baw_premium(a) + 1e-4 * o["k"]) * b5
+    vb = ctx.bs_price(eb) + baw_premium(b) + (baw_premium(b) + 1e-4 * o["k"]) * b5
+    want = ((x1 - xs) * va + (xs - x0) * vb) / (x1 - x0) - ctx.bs_price(es)
+    d = np.abs(pm[ok] - want[ok])
+    _say("  node form vs restatement: max", d.max())
+    assert d.max() <= 1e-9 * max(1.0, np.abs(want[ok]).max())
     s.close()
 
 
@@ -216,6 +277,7 @@ def test_serving_routes_and_fallback(ctx, tmp_path):
     d_o = torch.from_numpy(o.view(np.uint8).copy()).cuda()
     s = kwgpu.Surrogate(ctx, members=2, seed=4)
     fd = kwgpu.params(256, 64)
+    s.fit(d_o.data_ptr(), n, fd, epochs=1, batch=1024)  # the configuration it emulates
     d_p = torch.empty(n, dtype=torch.float64, device="cuda")
     d_r = torch.empty(n, dtype=torch.int32, device="cuda")
     # nothing passes a negative spread bound: every option is priced by FD
@@ -236,8 +298,11 @@ def test_serving_routes_and_fallback(ctx, tmp_path):
     assert nfd == half
     assert np.max(np.abs(d_p.cpu().numpy()[:half] - ref[:half])) <= 1e-4
     # everything in the box passes an infinite bound; out-of-box options still go to FD
-    feats = kwgpu.surrogate_features(o)
-    in_box = np.all(np.abs(feats) <= 1 + 1e-9, axis=1)
+    feats = kwgpu.surrogate_features(o, 256, 64)
+    _, _, ib = _predict(s, d_o, n, fd)
+    in_box = ib == 1
+    box6 = np.all(np.abs(feats[:, :6]) <= 1 + 1e-9, axis=1)
+    assert np.all(box6[in_box]) and in_box.sum() >= box6.sum() - 4 and 0 < in_box.sum() < n
     nfd = s.price(d_o.data_ptr(), n, 1e300, fd, d_p.data_ptr(), d_r.data_ptr())
     route = d_r.cpu().numpy()
     assert nfd == int((~in_box).sum())
@@ -251,6 +316,9 @@ def test_serving_routes_and_fallback(ctx, tmp_path):
     d_p2 = torch.empty(n, dtype=torch.float64, device="cuda")
     s2.price(d_o.data_ptr(), n, 1e300, fd, d_p2.data_ptr(), d_r.data_ptr())
     assert np.array_equal(d_p2.cpu().numpy()[in_box], p[in_box])
+    # another FD configuration than the emulated one: nothing is served
+    fd2 = kwgpu.params(512, 64)
+    assert s.price(d_o.data_ptr(), n, 1e300, fd2, d_p.data_ptr(), d_r.data_ptr()) == n
     s.close()
     s2.close()
 
@@ -269,6 +337,8 @@ def test_serving_per_option_grids(ctx):
     d_x = torch.from_numpy(xd.copy()).cuda()
     d_t = torch.from_numpy(td.copy()).cuda()
     s = kwgpu.Surrogate(ctx, members=2, seed=5)
+    s.fit(d_o.data_ptr(), n, kwgpu.params(512, 512, x_dims=d_x.data_ptr(), t_dims=d_t.data_ptr()),
+          epochs=1, batch=1024)  # a surrogate of per-option grids
     d_p = torch.empty(n, dtype=torch.float64, device="cuda")
     d_r = torch.empty(n, dtype=torch.int32, device="cuda")
     ref, st, rc, _ = ctx.fd1d_price(o, 512, 512, x_dims=xd, t_dims=td, precision=kwgpu.FP64)
@@ -282,8 +352,8 @@ def test_serving_per_option_grids(ctx):
         assert s.price(d_o.data_ptr(), n, -1.0, fd, d_p.data_ptr(), d_r.data_ptr()) == n
         assert np.max(np.abs(d_p.cpu().numpy() - ref)) <= tol
         # only the out-of-box options to FD: a scattered subset keeps its own sizes
-        feats = kwgpu.surrogate_features(o)
-        out = ~np.all(np.abs(feats) <= 1 + 1e-9, axis=1)
+        _, _, ib = _predict(s, d_o, n, fd)
+        out = ib == 0
         assert 0 < out.sum() < n
         nfd = s.price(d_o.data_ptr(), n, 1e300, fd, d_p.data_ptr(), d_r.data_ptr())
         assert nfd == int(out.sum())

@@ -61,7 +61,7 @@ def parse_args():
                     help="training options, each labelled at its two bracketing mesh nodes "
                          "(FD FP64 1024x1024); 5 %% of them are held out")
     ap.add_argument("--sur-serve", type=int, default=1 << 20)
-    ap.add_argument("--sur-epochs", type=int, default=160)
+    ap.add_argument("--sur-epochs", type=int, default=320)
     ap.add_argument("--sur-per-option", type=int, default=1,
                     help="also fit a surrogate to the per-option-grid FD pricer and serve "
                          "the batch on per-option grids")
@@ -320,28 +320,30 @@ def bench_surrogate(ctx, args, log):
     sur, train = pipeline.train_and_evaluate(ctx, n_train=args.sur_train - n_test, n_test=n_test,
                                              epochs=args.sur_epochs, log=log)
     opts = W.serving(args.sur_serve, 4)
-    bound = train["spread_bound"]
-    serve = pipeline.serve(ctx, sur, opts, bound, log=log)
+    bound, smax = train["spread_bound"], train["scale_bound"]
+    serve = pipeline.serve(ctx, sur, opts, bound, log=log, scale_max=smax)
     serve["meets_bar"] = serve["max_abs_err_vs_fp64_fd"] <= pipeline.ERR_BAR
-    serve32 = pipeline.serve(ctx, sur, opts, bound, log=log, precision=kwgpu.FP32_MIXED)
+    serve32 = pipeline.serve(ctx, sur, opts, bound, log=log, precision=kwgpu.FP32_MIXED,
+                             scale_max=smax)
     serve32["meets_bar"] = serve32["max_abs_err_vs_fp64_fd"] <= pipeline.ERR_BAR
     # the trade-off at looser bars (not the metric: its bar is 1e-4)
     looser = {}
     for bar, v in train["looser_bars"].items():
-        r = pipeline.serve(ctx, sur, opts, v["spread_bound"], repeats=1, log=log)
+        r = pipeline.serve(ctx, sur, opts, v["spread_bound"], repeats=1, log=log,
+                           scale_max=v["scale_bound"])
         r["meets_bar"] = r["max_abs_err_vs_fp64_fd"] <= float(bar)
         looser[bar] = r
     sur.close()
     uneven = {}
     if args.sur_per_option:
         sur2, train2 = pipeline.train_and_evaluate(ctx, n_train=args.sur_train - n_test,
-                                                   n_test=n_test, epochs=args.sur_epochs,
+                                                   n_test=n_test, epochs=args.sur_epochs // 2,
                                                    per_option_grids=True, log=log)
         xd, td = W.serving_grids(opts, 5)
         uneven["training"] = train2
         for name, prec in (("fp64", kwgpu.FP64), ("fp32_mixed", kwgpu.FP32_MIXED)):
             r = pipeline.serve(ctx, sur2, opts, train2["spread_bound"], log=log, x_dims=xd,
-                               t_dims=td, precision=prec)
+                               t_dims=td, precision=prec, scale_max=train2["scale_bound"])
             r["meets_bar"] = r["max_abs_err_vs_fp64_fd"] <= pipeline.ERR_BAR
             uneven[name + "_fallback"] = r
         sur2.close()

@@ -287,7 +287,10 @@ int kwgpu_surrogate_predict(kwgpu_ctx* ctx, kwgpu_surrogate* s, const kwgpu_opti
                             double* d_spread, uint8_t* d_in_box, void* stream);
 
 /* Surrogate-first pricing: options in the box whose ensemble spread is <= spread_max
- * are priced by the surrogate (BS + premium); the rest go to the FD pricer with `fd`
+ * are priced by the surrogate (BS + premium); the rest go to the FD pricer with `fd`.
+ * The ensemble runs only on candidates: in-box options with a labelled bracket whose
+ * baseline scale (baseline premium + 1e-4 K, the price scale of the network's error) is
+ * at most scale_max (INFINITY: every in-box option); the others go to FD directly
  * (fd->precision FP64, or FP32_MIXED with options whose exp(scale sigma sqrt(T)) would
  * overflow FP32 sent to FP64). fd->x_dims / fd->t_dims, when set, are device arrays of
  * n per-option grid sizes, and each routed option is solved on its own. The surrogate
@@ -301,8 +304,9 @@ int kwgpu_surrogate_predict(kwgpu_ctx* ctx, kwgpu_surrogate* s, const kwgpu_opti
  * synchronous (the FD subset is compacted on the device). Per-option work buffers are
  * kept with `s` and grown on demand. */
 int kwgpu_surrogate_price(kwgpu_ctx* ctx, kwgpu_surrogate* s, const kwgpu_option* d_opts,
-                          size_t n, double spread_max, const kwgpu_fd1d_params* fd,
-                          double* d_prices, int32_t* d_route, size_t* n_fd);
+                          size_t n, double spread_max, double scale_max,
+                          const kwgpu_fd1d_params* fd, double* d_prices, int32_t* d_route,
+                          size_t* n_fd);
 
 /* Weights of one member as FP32 (KWGPU_SURROGATE_PARAMS values, host memory). */
 int kwgpu_surrogate_get_params(kwgpu_ctx* ctx, const kwgpu_surrogate* s, int32_t member,

@@ -458,18 +458,26 @@ __device__ __forceinline__ void setup(const FdBatch& b, const Pde& d, const Lane
 }
 
 // All segments of a workgroup iterate to its longest march; lanes whose own march is
-// done keep their values.
+// done keep their values. *wmin (optional) receives the shortest march among segments
+// that march at all: up to there every segment steps, so no lane needs its own test
+// (segments with no march are identity rows, which stepping leaves unchanged).
 template <int M, int L, int W>
-__device__ __forceinline__ int group_steps(const Pde& d, const Lanes& t, double* qp) {
+__device__ __forceinline__ int group_steps(const Pde& d, const Lanes& t, double* qp,
+                                           int* wmin = nullptr) {
     using Gm = Geo<M, L, W>;
     if constexpr (Gm::G == 1) {
+        if (wmin) *wmin = d.steps;
         return d.steps;
     } else {
         __syncthreads();
         qp[t.gl] = (double)d.steps;
         __syncthreads();
-        double mx = 0.0;
-        for (int k = 0; k < Gm::NT; k += Gm::LT) mx = fmax(mx, qp[k]);
+        double mx = 0.0, mn = 1e300;
+        for (int k = 0; k < Gm::NT; k += Gm::LT) {
+            mx = fmax(mx, qp[k]);
+            if (qp[k] > 0.0) mn = fmin(mn, qp[k]);
+        }
+        if (wmin) *wmin = mn < 1e300 ? (int)mn : 0;
         return (int)mx;
     }
 }
@@ -621,45 +629,34 @@ __device__ __forceinline__ void interpolate(const FdBatch& b, const Pde& d, cons
 
 // ---- FP64 march (src/Math/kwFd1d.cpp:67-133) -----------------------------------------
 // One step of the scheme on the state xv; HALF: theta = 1/2, where the correction is
-// a single FMA a_j w_j - V_j.
+// a single FMA a_j w_j - V_j. Steps [0, wmin) are taken by every segment of the
+// workgroup, steps [wmin, wsteps) only by segments whose own march (`steps`) lasts that
+// long (several options per workgroup).
 template <int M, int L, int W, bool HALF>
 __device__ __forceinline__ void march_f64(double (&xv)[M], const double (&ap)[M],
                                           const double (&lo)[M], const double (&up)[M],
                                           const double (&R)[M], const double2* qpv,
                                           const ScanMul<double>& sm, const Lanes& t,
                                           bool first_lane, bool last_lane, double* xchg,
-                                          int wsteps, int steps, double c_th) {
+                                          int wmin, int wsteps, int steps, double c_th) {
     constexpr int NT = Geo<M, L, W>::NT;
-    // One option per workgroup (every lane marches every step): the loop is rotated so
-    // that each row's update is followed at once by that row's forward substitution for
-    // the next step. The substitution is a dependent chain of M FMAs; the row updates
-    // are independent of each other, so interleaved they fill the chain's latency.
-    constexpr bool ROT = Geo<M, L, W>::G == 1;
+    constexpr bool ONE = Geo<M, L, W>::G == 1;  // one option per workgroup
     // LDS reads of the (Q, payoff) pairs in flight ahead of the row updates. Measured on
     // one box: the 1024-point one-wave kernel 9.81 ms at 3, 10.03 at 4, 10.06 at 1 (and
     // 10.3 at 0); the 2048-point two-wave kernel 13.2 ms at 0, 15.6 at 1, 22.3 at 3,
     // where the extra registers turn into spill reloads inside the step loop; the
     // 256- and 512-point kernels (4 and 2 options per wave) 2.69 and 5.34 ms at 0, 2.88
     // and 6.33 at 3.
-    constexpr int PF0 = W == 2 ? KWGPU_PF2 : ROT ? KWGPU_PF1 : KWGPU_PFG;
+    constexpr int PF0 = W == 2 ? KWGPU_PF2 : ONE ? KWGPU_PF1 : KWGPU_PFG;
     constexpr int PF = PF0 < M ? PF0 : M - 1;  // 0: each pair read where it is used
     const int wv = __builtin_amdgcn_readfirstlane(t.wv);  // wave of a W = 2 group
     double g[M];
-    if constexpr (ROT) {
-        g[0] = xv[0];
-#pragma unroll
-        for (int i = 1; i < M; ++i) g[i] = fma(-lo[i], g[i - 1], xv[i]);
-    }
-    for (int step = 0; step < wsteps; ++step) {
-        if constexpr (!ROT) {
-            g[0] = xv[0];
-#pragma unroll
-            for (int i = 1; i < M; ++i) g[i] = fma(-lo[i], g[i - 1], xv[i]);
-        }
+    // backward substitution of g in place and the step's carries: g_in (forward) and
+    // w_in (backward) of this lane
+    auto carries = [&](int step, double& gin, double& win) {
         const double c = g[M - 1];
 #pragma unroll
         for (int i = M - 2; i >= 0; --i) g[i] = fma(-up[i], g[i + 1], g[i]);
-        double gin, win;
         if constexpr (W == 2) {
             // both waves scan their own rows, then meet once: wave 0 publishes its last
             // forward carry c0, wave 1 its first backward carry with c0 left out; the
@@ -682,48 +679,60 @@ __device__ __forceinline__ void march_f64(double (&xv)[M], const double (&ap)[M]
             gin = fwd_carry<L, W>(c, sm, t, first_lane, xchg);
             win = bwd_carry<L, W>(fma(R[0], gin, g[0]), sm, t, last_lane, xchg);
         }
-        if constexpr (ROT) {
-            // no branch, so the LDS reads and fix-ups can also move into the backward
-            // scan; the (Q, payoff) pairs are read PF rows ahead of their use
-            double2 qbuf[PF > 0 ? PF : 1];
+    };
+    // Steps every segment takes: no per-lane test, so the loop is rotated. Each row's
+    // update is followed at once by that row's forward substitution for the next step:
+    // the substitution is a dependent chain of M FMAs, the row updates are independent
+    // of each other, so interleaved they fill the chain's latency.
+    if (wmin > 0) {
+        g[0] = xv[0];
 #pragma unroll
-            for (int k = 0; k < PF; ++k) qbuf[k] = qpv[k * NT + t.gl];
+        for (int i = 1; i < M; ++i) g[i] = fma(-lo[i], g[i - 1], xv[i]);
+    }
+    for (int step = 0; step < wmin; ++step) {
+        double gin, win;
+        carries(step, gin, win);
+        // no branch, so the LDS reads and fix-ups can also move into the backward scan;
+        // the (Q, payoff) pairs are read PF rows ahead of their use
+        double2 qbuf[PF > 0 ? PF : 1];
 #pragma unroll
-            for (int i = 0; i < M; ++i) {
-                double2 qv;
-                if constexpr (PF == 0) {
-                    qv = qpv[i * NT + t.gl];
-                } else {
-                    qv = qbuf[0];
+        for (int k = 0; k < PF; ++k) qbuf[k] = qpv[k * NT + t.gl];
 #pragma unroll
-                    for (int k = 0; k + 1 < PF; ++k) qbuf[k] = qbuf[k + 1];
-                    if (i + PF < M) qbuf[PF - 1] = qpv[(i + PF) * NT + t.gl];
-                }
-                double w = fma(R[i], gin, g[i]);
-                w = fma(qv.x, win, w);
-                const double v = HALF ? fma(ap[i], w, -xv[i]) : fma(ap[i], w, -c_th * xv[i]);
-                xv[i] = vmax(v, qv.y);
-                g[i] = i == 0 ? xv[0] : fma(-lo[i], g[i - 1], xv[i]);  // next step's
+        for (int i = 0; i < M; ++i) {
+            double2 qv;
+            if constexpr (PF == 0) {
+                qv = qpv[i * NT + t.gl];
+            } else {
+                qv = qbuf[0];
+#pragma unroll
+                for (int k = 0; k + 1 < PF; ++k) qbuf[k] = qbuf[k + 1];
+                if (i + PF < M) qbuf[PF - 1] = qpv[(i + PF) * NT + t.gl];
             }
-        } else if (step < steps) {
-            double2 qbuf[PF > 0 ? PF : 1];
+            double w = fma(R[i], gin, g[i]);
+            w = fma(qv.x, win, w);
+            const double v = HALF ? fma(ap[i], w, -xv[i]) : fma(ap[i], w, -c_th * xv[i]);
+            xv[i] = vmax(v, qv.y);
+            g[i] = i == 0 ? xv[0] : fma(-lo[i], g[i - 1], xv[i]);  // next step's
+        }
+    }
+    // Steps only the longer segments take (several options per workgroup): lanes whose
+    // own march is done keep their values.
+    if constexpr (!ONE) {
+        for (int step = wmin; step < wsteps; ++step) {
+            g[0] = xv[0];
 #pragma unroll
-            for (int k = 0; k < PF; ++k) qbuf[k] = qpv[k * NT + t.gl];
+            for (int i = 1; i < M; ++i) g[i] = fma(-lo[i], g[i - 1], xv[i]);
+            double gin, win;
+            carries(step, gin, win);
+            if (step < steps) {
 #pragma unroll
-            for (int i = 0; i < M; ++i) {
-                double2 qv;
-                if constexpr (PF == 0) {
-                    qv = qpv[i * NT + t.gl];
-                } else {
-                    qv = qbuf[0];
-#pragma unroll
-                    for (int k = 0; k + 1 < PF; ++k) qbuf[k] = qbuf[k + 1];
-                    if (i + PF < M) qbuf[PF - 1] = qpv[(i + PF) * NT + t.gl];
+                for (int i = 0; i < M; ++i) {
+                    const double2 qv = qpv[i * NT + t.gl];
+                    double w = fma(R[i], gin, g[i]);
+                    w = fma(qv.x, win, w);
+                    const double v = HALF ? fma(ap[i], w, -xv[i]) : fma(ap[i], w, -c_th * xv[i]);
+                    xv[i] = vmax(v, qv.y);
                 }
-                double w = fma(R[i], gin, g[i]);
-                w = fma(qv.x, win, w);
-                const double v = HALF ? fma(ap[i], w, -xv[i]) : fma(ap[i], w, -c_th * xv[i]);
-                xv[i] = vmax(v, qv.y);
             }
         }
     }
@@ -745,7 +754,9 @@ __device__ __forceinline__ void solve_group_f64(const FdBatch& b, int group, dou
     const bool first_lane = t.ls == 0;
     const bool last_lane = t.ls == Gm::LT - 1;
     // the same for every lane of the workgroup: keep the loop counter scalar
-    const int wsteps = __builtin_amdgcn_readfirstlane(group_steps<M, L, W>(d, t, qp));
+    int wmin = 0;
+    const int wsteps = __builtin_amdgcn_readfirstlane(group_steps<M, L, W>(d, t, qp, &wmin));
+    wmin = __builtin_amdgcn_readfirstlane(wmin);
     double aw = 0.0, bw = 0.0;
     sm.TW0 = 0.0;
     if constexpr (W == 2) {
@@ -781,8 +792,8 @@ __device__ __forceinline__ void solve_group_f64(const FdBatch& b, int group, dou
     __syncthreads();
     const int steps = d.steps;
     // (the general form gives the same values at theta = 1/2: -c_th V is exactly -V)
-    march_f64<M, L, W, !GTH>(xv, ap, lo, up, R, qpv, sm, t, first_lane, last_lane, xchg, wsteps,
-                             steps, c_th);
+    march_f64<M, L, W, !GTH>(xv, ap, lo, up, R, qpv, sm, t, first_lane, last_lane, xchg, wmin,
+                             wsteps, steps, c_th);
     // the option's record and lane indices are derived again rather than kept in
     // registers over the march: the group and thread index go through an empty asm so
     // that the compiler cannot reuse the set-up's values (which it would otherwise keep,

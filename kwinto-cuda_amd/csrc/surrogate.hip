@@ -795,6 +795,56 @@ __global__ __launch_bounds__(kGateBlock) void compact_routes(const int32_t* rout
     }
 }
 
+// Serving candidates: in the training box, with a labelled bracket, and a baseline scale
+// baw + 1e-4 K (the price scale of the network's error) of at most scale_max; flag[i] = 1
+// for a candidate, else 0, with per-block counts as gate_kernel keeps them (so that
+// compact_routes lists the candidates in index order).
+__global__ __launch_bounds__(kGateBlock) void candidate_kernel(const kwgpu_option* opts, size_t n,
+                                                               GridArg g, double scale_max,
+                                                               int32_t* flag, uint32_t* blk_counts) {
+    __shared__ uint32_t wc[kGateBlock / 64];
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int c = 0;
+    if (i < n) {
+        const kwgpu_option o = opts[i];
+        float f[F];
+        const bool in = option_features(o, g.x(i), g.t(i), f);
+        if (in && node_pair(o, g.x(i), g.t(i), g.density, g.scale).ok)
+            c = baw_premium(o) + kEps * o.k <= scale_max ? 1 : 0;
+        flag[i] = c;
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint64_t v = __ballot(c == 1);
+    if (lane == 0) wc[w] = (uint32_t)__popcll(v);
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        uint32_t t = 0;
+        if (threadIdx.x == 0)
+            for (int q = 0; q < kGateBlock / 64; ++q) t += wc[q];
+        blk_counts[2 * blockIdx.x + threadIdx.x] = t;
+    }
+}
+
+// No prediction (premium NaN, in_box 0) for every option; then the candidates' own.
+__global__ void clear_prediction(size_t n, double* premium, double* spread, uint8_t* in_box) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    premium[i] = NAN;
+    spread[i] = INFINITY;
+    in_box[i] = 0;
+}
+
+__global__ void scatter_prediction(const uint32_t* idx, int nc, const double* pc, const double* sc,
+                                   const uint8_t* ic, double* premium, double* spread,
+                                   uint8_t* in_box) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nc) return;
+    const uint32_t i = idx[k];
+    premium[i] = pc[k];
+    spread[i] = sc[k];
+    in_box[i] = ic[k];
+}
+
 __global__ void gather_opts(const kwgpu_option* opts, const uint32_t* idx, int n, kwgpu_option* out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) out[i] = opts[idx[i]];
@@ -1496,8 +1546,9 @@ int kwgpu_surrogate_predict(kwgpu_ctx* ctx, kwgpu_surrogate* s, const kwgpu_opti
 }
 
 int kwgpu_surrogate_price(kwgpu_ctx* ctx, kwgpu_surrogate* s, const kwgpu_option* d_opts,
-                          size_t n, double spread_max, const kwgpu_fd1d_params* fd,
-                          double* d_prices, int32_t* d_route, size_t* n_fd) {
+                          size_t n, double spread_max, double scale_max,
+                          const kwgpu_fd1d_params* fd, double* d_prices, int32_t* d_route,
+                          size_t* n_fd) {
     if (!s || !d_opts || !d_prices || !d_route || !fd)
         return kwgpu::ctx_fail(ctx, KWGPU_ERR_INVALID, "kwgpu_surrogate_price: bad argument");
     if (n_fd) *n_fd = 0;
@@ -1514,7 +1565,7 @@ int kwgpu_surrogate_price(kwgpu_ctx* ctx, kwgpu_surrogate* s, const kwgpu_option
     size_t need = 0;
     for (size_t b : {sizeof(double), sizeof(double), (size_t)1, sizeof(uint32_t), sizeof(double),
                      sizeof(int32_t), sizeof(kwgpu_option), sizeof(int32_t), sizeof(int32_t),
-                     sizeof(uint32_t)})
+                     sizeof(uint32_t), sizeof(double), sizeof(double), (size_t)1})
         need += (b * n + 255) / 256 * 256;
     need += 256;  // the two counters
     if (s->serve_cap < n) {
@@ -1538,23 +1589,56 @@ int kwgpu_surrogate_price(kwgpu_ctx* ctx, kwgpu_surrogate* s, const kwgpu_option
     kwgpu_option* sub_o = (kwgpu_option*)carve(base, at, sizeof(kwgpu_option));
     int32_t* sub_x = (int32_t*)carve(base, at, sizeof(int32_t));
     int32_t* sub_t = (int32_t*)carve(base, at, sizeof(int32_t));
+    double* prem_c = (double*)carve(base, at, sizeof(double));  // the candidates' predictions
+    double* spr_c = (double*)carve(base, at, sizeof(double));
+    uint8_t* inb_c = (uint8_t*)carve(base, at, 1);
     int* cnt = (int*)(base + at);
     int rc = KWGPU_OK;
     auto cleanup = [&]() { (void)hipStreamSynchronize(st); };
     // a negative (or NaN) bound passes no option, and neither does an FD configuration
     // other than the one the surrogate emulates: the ensemble is not evaluated
     const bool serve_none = !(spread_max >= 0.0) || !grid_matches(s, fd);
+    const unsigned nb = nblk(n, kGateBlock);
     if (!serve_none) {
-        rc = kwgpu_surrogate_predict(ctx, s, d_opts, n, fd, prem, spr, inb, st);
-        if (rc) {
+        // the ensemble runs on the candidates only: listed in index order, gathered with
+        // their grid sizes, predicted, and their predictions scattered back
+        const GridArg g = grid_arg(s, fd);
+        hipLaunchKernelGGL(candidate_kernel, dim3(nb), dim3(kGateBlock), 0, st, d_opts, n, g,
+                           scale_max, d_route, blk);
+        hipLaunchKernelGGL(scan_block_counts, dim3(1), dim3(1024), 0, st, blk, nb, cnt);
+        hipLaunchKernelGGL(compact_routes, dim3(nb), dim3(kGateBlock), 0, st, d_route, n, blk,
+                           (const int*)cnt, idx);
+        hipLaunchKernelGGL(clear_prediction, dim3(nblk(n, 256)), dim3(256), 0, st, n, prem, spr, inb);
+        int nc = 0;
+        if (hipMemcpyAsync(&nc, cnt, sizeof(int), hipMemcpyDeviceToHost, st) ||
+            hipStreamSynchronize(st)) {
             cleanup();
-            return rc;
+            return kwgpu::ctx_fail(ctx, KWGPU_ERR_HIP, "kwgpu_surrogate_price: candidates");
+        }
+        if (nc > 0) {
+            hipLaunchKernelGGL(gather_opts, dim3(nblk(nc, 256)), dim3(256), 0, st, d_opts, idx, nc,
+                               sub_o);
+            kwgpu_fd1d_params pc = *fd;
+            if (fd->x_dims) {
+                hipLaunchKernelGGL(gather_dims, dim3(nblk(nc, 256)), dim3(256), 0, st, fd->x_dims,
+                                   idx, nc, sub_x);
+                hipLaunchKernelGGL(gather_dims, dim3(nblk(nc, 256)), dim3(256), 0, st, fd->t_dims,
+                                   idx, nc, sub_t);
+                pc.x_dims = sub_x;
+                pc.t_dims = sub_t;
+            }
+            rc = kwgpu_surrogate_predict(ctx, s, sub_o, (size_t)nc, &pc, prem_c, spr_c, inb_c, st);
+            if (rc) {
+                cleanup();
+                return rc;
+            }
+            hipLaunchKernelGGL(scatter_prediction, dim3(nblk(nc, 256)), dim3(256), 0, st, idx, nc,
+                               prem_c, spr_c, inb_c, prem, spr, inb);
         }
     }
     // FP32-mixed fallback: exp(scale * sigma sqrt(T)) must stay below the FP32 range
     // (~3.4e38 = e^88.7), so such options go to FP64; an FP64 fallback takes them all
     const double fp64_above = fd->precision == KWGPU_FP64 ? 0.0 : 80.0 / fd->scale;
-    const unsigned nb = nblk(n, kGateBlock);
     hipLaunchKernelGGL(gate_kernel, dim3(nb), dim3(kGateBlock), 0, st, d_opts, n,
                        serve_none ? nullptr : prem, spr, inb, spread_max, fp64_above, d_prices,
                        d_route, blk);

@@ -100,7 +100,7 @@ def lib():
                                           ctypes.POINTER(SurrogateTrainParams), P,
                                           ctypes.POINTER(ctypes.c_size_t)]
         L.kwgpu_surrogate_predict.argtypes = [P, P, P, SZ, ctypes.POINTER(Fd1dParams), P, P, P, P]
-        L.kwgpu_surrogate_price.argtypes = [P, P, P, SZ, ctypes.c_double,
+        L.kwgpu_surrogate_price.argtypes = [P, P, P, SZ, ctypes.c_double, ctypes.c_double,
                                             ctypes.POINTER(Fd1dParams), P, P,
                                             ctypes.POINTER(ctypes.c_size_t)]
         L.kwgpu_surrogate_get_params.argtypes = [P, P, ctypes.c_int32, P]
@@ -354,14 +354,15 @@ class Surrogate:
                                                   d_premium, d_spread, d_in_box, stream))
 
     def price(self, d_opts: int, n: int, spread_max: float, fd: Fd1dParams, d_prices: int,
-              d_route: int) -> int:
+              d_route: int, scale_max: float = float("inf")) -> int:
         """Surrogate-first prices into d_prices; d_route[i] = 0 (surrogate), 1 (FD at
         fd.precision) or 2 (FD in FP64). A negative spread_max serves nothing and skips
-        the ensemble. fd.x_dims / fd.t_dims may hold device arrays of per-option grid
-        sizes. Returns the number of options priced by FD."""
+        the ensemble; options whose baseline scale exceeds scale_max skip it too.
+        fd.x_dims / fd.t_dims may hold device arrays of per-option grid sizes. Returns
+        the number of options priced by FD."""
         nfd = ctypes.c_size_t(0)
         self._check(lib().kwgpu_surrogate_price(self.ctx.h, self.h, d_opts, n, spread_max,
-                                                ctypes.byref(fd), d_prices, d_route,
+                                                scale_max, ctypes.byref(fd), d_prices, d_route,
                                                 ctypes.byref(nfd)))
         return nfd.value
 

@@ -21,7 +21,7 @@ import time
 
 import numpy as np
 
-from . import FP64, Context, Surrogate, params
+from . import FP64, Context, Surrogate, params, surrogate_baseline
 from . import workloads as W
 
 X_DIM = 1024
@@ -119,11 +119,56 @@ def calibrate_spread(err, spread, bar=ERR_BAR, cap=SPREAD_MAX, margin=0.5,
     return float(s[idx[-1]])
 
 
-def train_and_evaluate(ctx: Context, n_train=2_000_000, n_test=65536, members=4, epochs=160,
-                       batch=8192, seed=3, per_option_grids=False, log=print):
-    """Fits the ensemble to the FD pricer and evaluates it on held-out options against FD
-    prices at their spots (same configuration)."""
+SCALE_CAPS = (0.01, 0.015, 0.02, 0.03, 0.05, 0.1, float("inf"))
+
+
+def calibrate_gate(err, spread, scale, bar=ERR_BAR, cap=SPREAD_MAX, caps=SCALE_CAPS):
+    """(spread bound, scale bound) of the gate: an option is served when its baseline
+    scale (baseline premium + 1e-4 K, the price scale of the network's error) is at most
+    the scale bound and its ensemble spread at most the spread bound. For each candidate
+    scale cap the spread bound is calibrated (calibrate_spread) on the held-out options
+    under the cap; the cap serving the most held-out options wins (the smaller on ties).
+    The cap matters because the members share their biases: where the baseline premium
+    is large, a small spread does not bound the error. (-1, -1): serve nothing."""
+    best = (-1.0, -1.0, 0)
+    for c in caps:
+        m = scale <= c
+        if m.sum() < MIN_CALIBRATED:
+            continue
+        b = calibrate_spread(err[m], spread[m], bar=bar, cap=cap)
+        n = int(np.sum(m & (spread <= b))) if b >= 0 else 0
+        if n > best[2]:
+            best = (b, float(c), n)
+    return best[0], best[1]
+
+
+def heldout_errors(ctx: Context, sur: Surrogate, opts, d_x=None, d_t=None):
+    """(error, spread, scale) of the surrogate's prices against FD at the options' own
+    spots (same configuration), for the in-box options with a labelled bracket whose FD
+    price succeeded; scale = baseline premium + 1e-4 K."""
     torch = _torch()
+    n = len(opts)
+    d_o = to_device(opts)
+    fd = fd_config(d_x, d_t)
+    truth_d = torch.empty(n, dtype=torch.float64, device="cuda")
+    st_d = torch.empty(n, dtype=torch.int32, device="cuda")
+    ctx.fd1d_price_device(d_o.data_ptr(), n, truth_d.data_ptr(), st_d.data_ptr(), fd)
+    pm, sp, ib = predict(ctx, sur, d_o, n, fd)
+    bs = bs_euro(ctx, opts).cpu().numpy()
+    truth, st = truth_d.cpu().numpy(), st_d.cpu().numpy()
+    inb = (ib.cpu().numpy() == 1) & (st == 0)
+    err = (bs + pm.cpu().numpy()) - truth
+    scale = surrogate_baseline(ctx, opts) + 1e-4 * opts["k"]
+    return err[inb], sp.cpu().numpy()[inb], scale[inb], int(inb.sum())
+
+
+def train_and_evaluate(ctx: Context, n_train=2_000_000, n_test=65536, members=4, epochs=160,
+                       batch=8192, seed=3, per_option_grids=False, n_calibrate=1 << 20,
+                       log=print):
+    """Fits the ensemble to the FD pricer, reports its error on the held-out options, and
+    calibrates the gate on a separate set of `n_calibrate` options (seed + 1000) priced by
+    FD: as many as a serving batch, so that the bounds see the error tail a batch of that
+    size shows (a 100k-option calibration once admitted a 1.2e-4 error on a 1M batch)."""
     opts = W.surrogate_train(n_train + n_test, seed)
     dims = W.serving_grids(opts, seed + 100) if per_option_grids else (None, None)
     d_x, d_t = device_dims(len(opts), *dims)
@@ -139,46 +184,49 @@ def train_and_evaluate(ctx: Context, n_train=2_000_000, n_test=65536, members=4,
         f"(FD FP64, {grid}), {members} members x {epochs} epochs in {t_fit:.2f} s, "
         f"final loss {loss[-1].tolist()}")
     # held-out options at their own spots, against the FD price on the same grid
-    d_te = d_o[n_train * 56:]
-    fd_te = fd_config(None if d_x is None else d_x[n_train:], None if d_t is None else d_t[n_train:])
-    truth_d = torch.empty(n_test, dtype=torch.float64, device="cuda")
-    st_d = torch.empty(n_test, dtype=torch.int32, device="cuda")
-    ctx.fd1d_price_device(d_te.data_ptr(), n_test, truth_d.data_ptr(), st_d.data_ptr(), fd_te)
-    pm, sp, ib = predict(ctx, sur, d_te, n_test, fd_te)
-    te = opts[n_train:]
-    bs = bs_euro(ctx, te).cpu().numpy()
-    truth, st = truth_d.cpu().numpy(), st_d.cpu().numpy()
-    inb = (ib.cpu().numpy() == 1) & (st == 0)
-    err = (bs + pm.cpu().numpy()) - truth
-    spread = sp.cpu().numpy()
-    e_in, s_in = err[inb], spread[inb]
-    thr = calibrate_spread(e_in, s_in)
-    served = s_in <= thr
+    ho = slice(n_train, n_train + n_test)
+    e_in, s_in, c_in, n_in = heldout_errors(ctx, sur, opts[ho], None if d_x is None else d_x[ho],
+                                            None if d_t is None else d_t[ho])
+    # the gate, calibrated on its own set
+    cal = W.surrogate_train(n_calibrate, seed + 1000)
+    cdims = W.serving_grids(cal, seed + 1100) if per_option_grids else (None, None)
+    c_x, c_t = device_dims(len(cal), *cdims)
+    e_c, s_c, c_c, _ = heldout_errors(ctx, sur, cal, c_x, c_t)
+    thr, scale_max = calibrate_gate(e_c, s_c, c_c)
+    served = (s_c <= thr) & (c_c <= scale_max)
+    served_ho = (s_in <= thr) & (c_in <= scale_max)
     # the same calibration at looser error bars, for the accuracy / throughput trade-off
     looser = {}
     for bar in (1e-3, 1e-2):
-        tb = calibrate_spread(e_in, s_in, bar=bar, cap=bar / 2)
-        sv = s_in <= tb
-        looser[f"{bar:g}"] = {"spread_bound": tb, "heldout_pass_fraction": float(np.mean(sv)),
-                              "heldout_max_err_served":
-                                  float(np.max(np.abs(e_in[sv]))) if sv.any() else 0.0}
+        tb, tc = calibrate_gate(e_c, s_c, c_c, bar=bar, cap=bar / 2)
+        sv = (s_c <= tb) & (c_c <= tc)
+        looser[f"{bar:g}"] = {"spread_bound": tb, "scale_bound": tc,
+                              "calibration_pass_fraction": float(np.mean(sv)),
+                              "calibration_max_err_served":
+                                  float(np.max(np.abs(e_c[sv]))) if sv.any() else 0.0}
     out = {
         "grid": grid, "n_train": n_train, "n_labelled_pairs": int(pairs), "n_test": n_test,
-        "n_test_in_box": int(inb.sum()), "members": members, "epochs": epochs,
+        "n_test_in_box": n_in, "members": members, "epochs": epochs,
         "fit_s": t_fit, "final_loss": loss[-1].tolist(),
         "heldout_max_abs_err": float(np.max(np.abs(e_in))),
         "heldout_rms_abs_err": float(np.sqrt(np.mean(e_in * e_in))),
         "heldout_frac_within_bar": float(np.mean(np.abs(e_in) <= ERR_BAR)),
+        "n_calibration": int(len(e_c)),
         "spread_bound": thr,
-        "heldout_pass_fraction": float(np.mean(served)),
-        "heldout_max_err_served": float(np.max(np.abs(e_in[served]))) if served.any() else 0.0,
+        "scale_bound": scale_max,
+        "calibration_candidate_fraction": float(np.mean(c_c <= scale_max)),
+        "calibration_pass_fraction": float(np.mean(served)),
+        "calibration_max_err_served": float(np.max(np.abs(e_c[served]))) if served.any() else 0.0,
+        "heldout_pass_fraction": float(np.mean(served_ho)),
+        "heldout_max_err_served":
+            float(np.max(np.abs(e_in[served_ho]))) if served_ho.any() else 0.0,
         "looser_bars": looser,
     }
     return sur, out
 
 
 def serve(ctx: Context, sur: Surrogate, opts, spread_bound: float, repeats=3, log=print,
-          x_dims=None, t_dims=None, precision=FP64):
+          x_dims=None, t_dims=None, precision=FP64, scale_max=float("inf")):
     """Surrogate-first pricing of `opts` against plain FP64 FD of the same batch:
     options/s of both, max/RMS error of the served prices vs FP64 FD. `x_dims`/`t_dims`
     (host int32 arrays, optional) give every option its own grid, for the FD fallback and
@@ -199,13 +247,15 @@ def serve(ctx: Context, sur: Surrogate, opts, spread_bound: float, repeats=3, lo
         ctx.fd1d_price_device(d_o.data_ptr(), n, d_ref.data_ptr(), d_st.data_ptr(), fd64)
         ctx.synchronize()
 
-    sur.price(d_o.data_ptr(), n, spread_bound, fd, d_p.data_ptr(), d_r.data_ptr())  # warm-up
+    sur.price(d_o.data_ptr(), n, spread_bound, fd, d_p.data_ptr(), d_r.data_ptr(),
+              scale_max)  # warm-up
     torch.cuda.synchronize()
     ts = []
     nfd = 0
     for _ in range(repeats):
         t0 = time.perf_counter()
-        nfd = sur.price(d_o.data_ptr(), n, spread_bound, fd, d_p.data_ptr(), d_r.data_ptr())
+        nfd = sur.price(d_o.data_ptr(), n, spread_bound, fd, d_p.data_ptr(), d_r.data_ptr(),
+                        scale_max)
         torch.cuda.synchronize()
         ts.append(time.perf_counter() - t0)
     t_sur = min(ts)

@@ -291,8 +291,8 @@ int main(int argc, char** argv) {
                         }
                         // a smaller batch after a larger one reuses the serving buffers
                         for (size_t m : {n, n / 3})
-                            expect(kwgpu_surrogate_price(c, sx, o.data(), m, bound, &p, pr.data(),
-                                                         route.data(), &nfd),
+                            expect(kwgpu_surrogate_price(c, sx, o.data(), m, bound, 0.05, &p,
+                                                         pr.data(), route.data(), &nfd),
                                    KWGPU_OK, c, "surrogate price");
                     }
         kwgpu_surrogate_destroy(sg);

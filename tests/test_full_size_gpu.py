@@ -159,10 +159,13 @@ def test_training_heldout_error(fitted):
     assert all(np.isfinite(out["final_loss"]))
     assert out["heldout_rms_abs_err"] <= 3e-3
     assert out["heldout_frac_within_bar"] >= 0.3
-    # the calibrated gate serves a share of the held-out set, each within half the bar
-    assert 0 < out["spread_bound"] <= P.SPREAD_MAX
-    assert out["heldout_pass_fraction"] > 0.02
-    assert out["heldout_max_err_served"] <= 0.5 * BAR
+    # the gate, calibrated on its own 1M-option set, serves a share of it within half the
+    # bar, and of the held-out set within the bar
+    assert out["n_calibration"] > 1_000_000
+    assert 0 < out["spread_bound"] <= P.SPREAD_MAX and out["scale_bound"] > 0
+    assert out["calibration_pass_fraction"] > 0.02
+    assert out["calibration_max_err_served"] <= 0.5 * BAR
+    assert out["heldout_pass_fraction"] > 0.02 and out["heldout_max_err_served"] <= BAR
 
 
 def test_serving_1m_full_size(ctx, fitted):
@@ -180,7 +183,8 @@ def test_serving_1m_full_size(ctx, fitted):
     d_p = torch.empty(n, dtype=torch.float64, device="cuda")
     d_r = torch.empty(n, dtype=torch.int32, device="cuda")
     t0 = time.perf_counter()
-    nfd = sur.price(d_o.data_ptr(), n, out["spread_bound"], fd, d_p.data_ptr(), d_r.data_ptr())
+    nfd = sur.price(d_o.data_ptr(), n, out["spread_bound"], fd, d_p.data_ptr(), d_r.data_ptr(),
+                    out["scale_bound"])
     t_sur = time.perf_counter() - t0
     p, route = d_p.cpu().numpy(), d_r.cpu().numpy()
     served = route == 0

@@ -40,3 +40,31 @@ def test_cap_limits_the_bound():
     spread = np.linspace(0.0, 1.0, 5000)
     thr = pipeline.calibrate_spread(err, spread, bar=1e-4, cap=0.5)
     assert thr <= 0.5 and thr == np.max(spread[spread <= 0.5])
+
+
+def test_scale_cap_excludes_shared_biases():
+    """Where the baseline scale is large the members share a bias that the spread does
+    not show: the gate's scale cap must leave those options out, and the cap chosen is
+    the one serving the most held-out options within half the bar."""
+    rng = np.random.default_rng(3)
+    n = 100_000
+    scale = rng.uniform(0.0, 0.2, n)
+    spread = rng.uniform(0.0, 5e-5, n)
+    err = spread * rng.normal(0.0, 0.4, n)
+    biased = scale > 0.05
+    err[biased] += 4e-3 * scale[biased] * rng.choice([-1.0, 1.0], biased.sum())
+    thr, cap = pipeline.calibrate_gate(err, spread, scale, bar=1e-4, cap=5e-5)
+    assert 0 < thr <= 5e-5 and cap <= 0.05
+    served = (spread <= thr) & (scale <= cap)
+    assert served.sum() >= pipeline.MIN_CALIBRATED and not np.any(served & biased)
+    assert np.max(np.abs(err[served])) <= 0.5e-4
+    # without the cap a small spread would admit biased options beyond the margin
+    thr_all = pipeline.calibrate_spread(err, spread, bar=1e-4, cap=5e-5)
+    assert thr_all < thr
+
+
+def test_no_cap_with_enough_options_serves_nothing():
+    err = np.full(500, 1.0)
+    spread = np.zeros(500)
+    scale = np.zeros(500)
+    assert pipeline.calibrate_gate(err, spread, scale) == (-1.0, -1.0)

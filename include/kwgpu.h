@@ -138,6 +138,23 @@ int kwgpu_fd1d_price_device(kwgpu_ctx* ctx, const kwgpu_option* d_opts, size_t n
                             const kwgpu_fd1d_params* params, double* d_prices,
                             int32_t* d_status, void* stream);
 
+/*
+ * kwgpu_fd1d_price_device without any allocation (no counterpart in the reference: it is
+ * for callers that cannot allocate device memory, such as code under HIP graph capture,
+ * tight serving loops and harnesses that forbid hipMalloc). Same pricing, per-option
+ * grids included, on device-resident options, prices and status, asynchronous on
+ * `stream`; here params->x_dims / params->t_dims, when set, are HOST arrays of n sizes.
+ * The PDE order, the sizes and the bucket counters of the batch live in a fixed device
+ * scratch array of the library (12 MiB per device, held until the process ends): at most
+ * 1048576 options (more returns KWGPU_ERR_INVALID), wave layout only (x_dim <= 2048, else
+ * KWGPU_ERR_UNSUPPORTED). No graph and no side stream: the grid-size buckets run one
+ * after another on `stream`. Calls on one context are ordered; calls from different
+ * contexts on one GPU share the scratch and race when they overlap.
+ */
+int kwgpu_fd1d_price_device_noalloc(kwgpu_ctx* ctx, const kwgpu_option* d_opts, size_t n,
+                                    const kwgpu_fd1d_params* params, double* d_prices,
+                                    int32_t* d_status, void* stream);
+
 /* Closed-form European price -- replaces BlackScholes_Pricer::price
  * (src/Pricer/kwBlackScholes.cpp:13-50): NaN for American options. Host pointers. */
 int kwgpu_bs_price(kwgpu_ctx* ctx, const kwgpu_option* opts, size_t n, double* prices);

@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <map>
+#include <mutex>
 #include <cstdio>
 #include <cstring>
 #include <numeric>
@@ -44,6 +45,30 @@ struct Bucket {
 constexpr int kLaneCap = 1 << 30;
 constexpr int kSide = 4;  // side streams an uneven batch forks onto
 
+// Device scratch of kwgpu_fd1d_price_device_noalloc: one uneven batch's PDE order, its
+// per-option x and t points, and its bucket counters, for at most kNoAllocMaxOptions
+// options (12 MiB + 256 B, held by every device the library runs on until the process
+// ends). Host code never uses the name: it takes the device address from
+// hipGetSymbolAddress, once per device, when a context is created there.
+constexpr size_t kNoAllocMaxOptions = size_t(1) << 20;
+constexpr int kNoAllocCounters = 64;
+__device__ __align__(256) uint32_t scratch[3 * kNoAllocMaxOptions + kNoAllocCounters];
+constexpr int kMaxDevices = 64;
+uint32_t* g_scratch[kMaxDevices] = {};  // device address of `scratch`, per device
+std::mutex g_scratch_mu;
+
+// With `device` current: looks up and keeps its address of `scratch` (first call per
+// device only).
+hipError_t init_scratch(int device) {
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch[device]) return hipSuccess;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch[device] = static_cast<uint32_t*>(ptr);
+    return hipSuccess;
+}
 
 struct kwgpu_ctx {
     int device = 0;
@@ -732,6 +757,9 @@ int kwgpu_create(kwgpu_ctx** out, int device) {
     if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count)
         return KWGPU_ERR_HIP;
     if (hipSetDevice(device) != hipSuccess) return KWGPU_ERR_HIP;
+    // the allocation-free entry point's scratch on this device (a failure only disables
+    // that entry point, which then reports it)
+    (void)init_scratch(device);
     auto* c = new kwgpu_ctx();
     c->device = device;
     if (const char* g = std::getenv("KWGPU_NO_GRAPH")) c->use_graphs = g[0] == '0';
@@ -961,6 +989,108 @@ int kwgpu_fd1d_greeks(kwgpu_ctx* c, const kwgpu_option* opts, size_t n,
         }
     }
     return collect(c, all.data(), n9, &p9, st, "kwgpu_fd1d_greeks: ");
+}
+
+int kwgpu_fd1d_price_device_noalloc(kwgpu_ctx* c, const kwgpu_option* d_opts, size_t n,
+                                    const kwgpu_fd1d_params* p, double* d_prices,
+                                    int32_t* d_status, void* stream) {
+    if (!c) return KWGPU_ERR_INVALID;
+    c->err.clear();
+    if (int rc = check_params(c, p)) return rc;
+    if (n == 0) return KWGPU_OK;
+    if (!d_opts || !d_prices) return fail(c, KWGPU_ERR_INVALID, "kwgpu: null pointer");
+    if (n > kNoAllocMaxOptions)
+        return fail(c, KWGPU_ERR_INVALID,
+                    "kwgpu_fd1d_price_device_noalloc: " + std::to_string(n) +
+                        " options exceed its maximum of " + std::to_string(kNoAllocMaxOptions));
+    if (int rc = check_dims(c, p->x_dims, p->t_dims, n)) return rc;
+    const int32_t* hx = p->x_dims;  // host arrays here
+    const int32_t* ht = p->t_dims;
+    int mx = p->x_dim;
+    if (hx) mx = *std::max_element(hx, hx + n);
+    if (p->layout == KWGPU_LAYOUT_LANE_SOA || !kwgpu::wave_layout_fits(mx, p->precision))
+        return fail(c, KWGPU_ERR_UNSUPPORTED,
+                    "kwgpu_fd1d_price_device_noalloc: the wave layout only (x_dim <= 2048)");
+    if (hipError_t e = hipSetDevice(c->device); e != hipSuccess) return hip_fail(c, e, "hipSetDevice");
+    uint32_t* base = c->device < kMaxDevices ? g_scratch[c->device] : nullptr;
+    if (!base)
+        return fail(c, KWGPU_ERR_HIP, "kwgpu_fd1d_price_device_noalloc: no scratch on this device");
+    hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+    // the previous solve on this context may still read the scratch (calls from other
+    // contexts on this device are not ordered: they race)
+    if (c->done_stream && c->done_stream != s)
+        if (hipError_t e = hipStreamWaitEvent(s, c->ev_done, 0); e != hipSuccess)
+            return hip_fail(c, e, "kwgpu: order after the previous solve");
+    uint32_t* d_order = base;
+    int32_t* d_x = reinterpret_cast<int32_t*>(base + kNoAllocMaxOptions);
+    int32_t* d_t = reinterpret_cast<int32_t*>(base + 2 * kNoAllocMaxOptions);
+    int* d_counters = reinterpret_cast<int*>(base + 3 * kNoAllocMaxOptions);
+    FdBatch b{};
+    b.opts = d_opts;
+    b.m = (int)n;
+    b.x_dim = p->x_dim;
+    b.t_dim = p->t_dim;
+    b.density = p->density;
+    b.scale = p->scale;
+    b.theta = p->theta;
+    b.prices = d_prices;
+    b.status = d_status;
+    // buckets: (x-points rounded up to a power of two) in ascending order, PDEs by
+    // t-points descending within each (ties by index), as for uneven batches elsewhere
+    std::vector<Bucket> plan;
+    hipError_t e = hipSuccess;
+    if (!hx && !ht) {
+        const int cap = bucket_of(p->x_dim, p->precision, p->layout);
+        plan.push_back(Bucket{cap, choice_for(c, cap, p->precision, p->layout), 0, n, p->x_dim,
+                              p->precision});
+    } else {
+        auto gx = [&](size_t i) { return hx ? hx[i] : p->x_dim; };
+        auto gt = [&](size_t i) { return ht ? ht[i] : p->t_dim; };
+        std::vector<uint32_t> ord(n);
+        std::iota(ord.begin(), ord.end(), 0u);
+        std::vector<int> cap(n);
+        for (size_t i = 0; i < n; ++i) cap[i] = bucket_of(gx(i), p->precision, p->layout);
+        std::sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t z) {
+            return std::make_tuple(cap[a], -gt(a), a) < std::make_tuple(cap[z], -gt(z), z);
+        });
+        for (size_t k = 0; k < n;) {
+            size_t k1 = k;
+            int widest = 0;
+            while (k1 < n && cap[ord[k1]] == cap[ord[k]]) widest = std::max(widest, gx(ord[k1++]));
+            const int cp = cap[ord[k]];
+            plan.push_back(Bucket{cp, choice_for(c, cp, p->precision, p->layout), k, k1 - k,
+                                  widest, p->precision});
+            k = k1;
+        }
+        std::vector<int32_t> fx, ft;
+        if (!hx) fx.assign(n, p->x_dim), hx = fx.data();
+        if (!ht) ft.assign(n, p->t_dim), ht = ft.data();
+        if ((e = hipMemcpyAsync(d_order, ord.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, s)) ||
+            (e = hipMemcpyAsync(d_x, hx, n * sizeof(int32_t), hipMemcpyHostToDevice, s)) ||
+            (e = hipMemcpyAsync(d_t, ht, n * sizeof(int32_t), hipMemcpyHostToDevice, s)) ||
+            (e = hipStreamSynchronize(s)))  // the host arrays must outlive the copies
+            return hip_fail(c, e, "kwgpu_fd1d_price_device_noalloc: upload");
+        b.order = d_order;
+        b.x_dims = d_x;
+        b.t_dims = d_t;
+    }
+    if (plan.size() > (size_t)kNoAllocCounters)
+        return fail(c, KWGPU_ERR_INVALID, "kwgpu_fd1d_price_device_noalloc: too many buckets");
+    // a fill kernel, not a memset, clears the counters the persistent grids take work from
+    if ((e = kwgpu::launch_zero_counters(d_counters, (int)plan.size(), s)))
+        return hip_fail(c, e, "kwgpu_fd1d_price_device_noalloc: counter reset");
+    for (size_t i = 0; i < plan.size(); ++i) {
+        const Bucket& k = plan[i];
+        FdBatch bb = b;
+        if (bb.order) bb.order += k.begin;
+        bb.m = (int)k.count;
+        if ((e = kwgpu::launch_fd1d_wave_variant(bb, k.ch.variant, k.prec, s,
+                                                 k.ch.persist ? d_counters + i : nullptr)))
+            return hip_fail(c, e, "kwgpu_fd1d_price_device_noalloc: launch");
+    }
+    if ((e = hipEventRecord(c->ev_done, s))) return hip_fail(c, e, "kwgpu: solve event");
+    c->done_stream = s;
+    return KWGPU_OK;
 }
 
 }  // extern "C"

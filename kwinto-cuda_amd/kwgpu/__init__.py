@@ -24,7 +24,7 @@ LAYOUT_AUTO, LAYOUT_LANE_SOA, LAYOUT_WAVE_REG = 0, 1, 2
 
 EXPORTED_SYMBOLS = (
     "kwgpu_fd1d_params_default", "kwgpu_create", "kwgpu_destroy", "kwgpu_last_error",
-    "kwgpu_fd1d_price", "kwgpu_fd1d_price_device",
+    "kwgpu_fd1d_price", "kwgpu_fd1d_price_device", "kwgpu_fd1d_price_device_noalloc",
     "kwgpu_bs_price", "kwgpu_bs_price_device",
     "kwgpu_fd1d_bs_price", "kwgpu_fd1d_greeks", "kwgpu_synchronize", "kwgpu_stream",
     "kwgpu_surrogate_train_params_default", "kwgpu_surrogate_create", "kwgpu_surrogate_destroy",
@@ -81,6 +81,8 @@ def lib():
         L.kwgpu_last_error.restype = ctypes.c_char_p
         L.kwgpu_fd1d_price.argtypes = [P, P, SZ, ctypes.POINTER(Fd1dParams), P, P]
         L.kwgpu_fd1d_price_device.argtypes = [P, P, SZ, ctypes.POINTER(Fd1dParams), P, P, P]
+        L.kwgpu_fd1d_price_device_noalloc.argtypes = [P, P, SZ, ctypes.POINTER(Fd1dParams), P, P,
+                                                      P]
         L.kwgpu_bs_price.argtypes = [P, P, SZ, P]
         L.kwgpu_bs_price_device.argtypes = [P, P, SZ, P, P]
         L.kwgpu_fd1d_bs_price.argtypes = [P, P, SZ, ctypes.POINTER(Fd1dParams), P, P]
@@ -244,6 +246,14 @@ class Context:
                           p: Fd1dParams, stream: int | None = None):
         rc = lib().kwgpu_fd1d_price_device(self.h, d_opts, n, ctypes.byref(p), d_prices,
                                            d_status, stream)
+        if rc != OK:
+            raise KwgpuError(rc, self.last_error)
+
+    def fd1d_price_device_noalloc(self, d_opts: int, n: int, d_prices: int, d_status: int | None,
+                                  p: Fd1dParams, stream: int | None = None):
+        """kwgpu_fd1d_price_device_noalloc: p.x_dims / p.t_dims are host addresses here."""
+        rc = lib().kwgpu_fd1d_price_device_noalloc(self.h, d_opts, n, ctypes.byref(p), d_prices,
+                                                   d_status, stream)
         if rc != OK:
             raise KwgpuError(rc, self.last_error)
 

@@ -199,6 +199,34 @@ int main(int argc, char** argv) {
         hipStreamDestroy(sa);
         hipStreamDestroy(sb);
     }
+    {  // the allocation-free entry point: host sizes, two calls, the second smaller;
+       // uniform grids; the size limit and the lane layout refused
+        auto o = options(2500, 91);
+        Sizes s = sizes(2500, {64, 256, 512, 1024, 2048}, 32, 300, 9);
+        kwgpu_fd1d_params_default(&p);
+        p.x_dims = s.x.data();
+        p.t_dims = s.t.data();
+        std::vector<double> pr(o.size());
+        std::vector<int32_t> st(o.size());
+        expect(kwgpu_fd1d_price_device_noalloc(c, o.data(), o.size(), &p, pr.data(), st.data(),
+                                               nullptr),
+               KWGPU_OK, c, "noalloc uneven");
+        expect(kwgpu_fd1d_price_device_noalloc(c, o.data(), 700, &p, pr.data(), st.data(), nullptr),
+               KWGPU_OK, c, "noalloc uneven, smaller");
+        kwgpu_fd1d_params_default(&p);
+        p.x_dim = 1000;
+        p.t_dim = 50;
+        expect(kwgpu_fd1d_price_device_noalloc(c, o.data(), o.size(), &p, pr.data(), st.data(),
+                                               nullptr),
+               KWGPU_OK, c, "noalloc uniform");
+        p.x_dim = 3000;
+        expect(kwgpu_fd1d_price_device_noalloc(c, o.data(), 10, &p, pr.data(), st.data(), nullptr),
+               KWGPU_ERR_UNSUPPORTED, c, "noalloc above 2048 points");
+        p.x_dim = 512;
+        expect(kwgpu_fd1d_price_device_noalloc(c, o.data(), (size_t(1) << 20) + 1, &p, pr.data(),
+                                               st.data(), nullptr),
+               KWGPU_ERR_INVALID, c, "noalloc above its maximum");
+    }
     {  // PDE grouping: ten (spot, strike) pairs per PDE
         auto base = options(60, 7);
         std::vector<kwgpu_option> book;

@@ -59,6 +59,12 @@ void** __hipRegisterFatBinary(const void*) {
 void __hipRegisterFunction(void**, const void*, char*, const char*, unsigned int, uint3*, uint3*,
                            dim3*, dim3*, int*) {}
 void __hipUnregisterFatBinary(void**) {}
+void __hipRegisterVar(void**, void*, char*, const char*, int, size_t, int, int) {}
+// a __device__ variable's host shadow (full size in a host-only build) stands in for it
+hipError_t hipGetSymbolAddress(void** p, const void* sym) {
+    *p = const_cast<void*>(sym);
+    return hipSuccess;
+}
 static dim3 g_grid, g_block;
 static size_t g_shm;
 static hipStream_t g_cs;

@@ -137,11 +137,29 @@ __host__ __device__ inline bool option_features(const kwgpu_option& o, int x_dim
     return in;
 }
 
+// Crank-Nicolson does not smooth the payoff's kink at the strike: its highest-frequency
+// modes there are multiplied by (1 - 2 mu) / (1 + 2 mu) ~ -exp(-1 / mu) per step, mu =
+// (sigma^2 / 2) dt / dx^2 on the strike's cell, so after the n steps of the march they
+// keep exp(-n / mu) of their size. The FD prices near the strike then carry a grid-scale
+// oscillation that no smooth function of the option and grid follows. Such an option
+// gets no labels and is never served: n / mu must be at least kDampMin (e-foldings).
+// Every option of the training box has n / mu >= 10.9 on the 1024 x 1024 grid; per-option
+// grids of many x-points and few steps (2048 points, 365 steps a year) go below it.
+constexpr double kDampMin = 10.0;
+
+__device__ inline double kink_damping(const PdeParams& P, double density) {
+    const int jk = bracket(P, density, 0.0);
+    if (jk < 1 || jk >= P.x_dim) return 0.0;
+    const double dx = grid_x(P, density, jk) - grid_x(P, density, jk - 1);
+    const double mu = P.axx * P.dt / (dx * dx);
+    return (double)(P.t_dim - 1) / mu;
+}
+
 // The two mesh nodes that bracket log(S/K) on the option's grid (Fd1d::value's
 // bracket: the first node x_j >= log(S/K)), as options at those spots, with the
 // interpolation abscissae. ok is false where the network has no labels to stand on:
-// an invalid option, a spot outside the grid, or a bracket at the first cell (the FD
-// pricer rejects a spot at x_0, so no label exists there).
+// an invalid option, a spot outside the grid, a bracket at the first cell (the FD
+// pricer rejects a spot at x_0, so no label exists there), or an undamped kink (above).
 struct NodePair {
     kwgpu_option a, b;  // spots K e^{x_{j-1}}, K e^{x_j}
     double x0, x1, xs;
@@ -154,7 +172,7 @@ __device__ inline NodePair node_pair(const kwgpu_option& o, int x_dim, int t_dim
     const PdeParams P = pde_params(o, x_dim, t_dim, density, scale);
     q.xs = log(o.s / o.k);
     const int j = P.valid && isfinite(q.xs) ? bracket(P, density, q.xs) : 0;
-    q.ok = P.valid && j >= 2 && j < x_dim;
+    q.ok = P.valid && j >= 2 && j < x_dim && kink_damping(P, density) >= kDampMin;
     const int jj = q.ok ? j : 2;
     q.x0 = grid_x(P, density, jj - 1);
     q.x1 = grid_x(P, density, jj);

@@ -122,24 +122,50 @@ def calibrate_spread(err, spread, bar=ERR_BAR, cap=SPREAD_MAX, margin=0.5,
 SCALE_CAPS = (0.01, 0.015, 0.02, 0.03, 0.05, 0.1, float("inf"))
 
 
-def calibrate_gate(err, spread, scale, bar=ERR_BAR, cap=SPREAD_MAX, caps=SCALE_CAPS):
+def calibrate_gate(err, spread, scale, bar=ERR_BAR, cap=SPREAD_MAX, caps=SCALE_CAPS, cost=0.0):
     """(spread bound, scale bound) of the gate: an option is served when its baseline
     scale (baseline premium + 1e-4 K, the price scale of the network's error) is at most
     the scale bound and its ensemble spread at most the spread bound. For each candidate
     scale cap the spread bound is calibrated (calibrate_spread) on the held-out options
-    under the cap; the cap serving the most held-out options wins (the smaller on ties).
-    The cap matters because the members share their biases: where the baseline premium
-    is large, a small spread does not bound the error. (-1, -1): serve nothing."""
-    best = (-1.0, -1.0, 0)
+    under the cap. The cap matters because the members share their biases: where the
+    baseline premium is large, a small spread does not bound the error.
+    Every option under the cap (a candidate) is run through the ensemble, which costs
+    `cost` FD solves (inference_cost_ratio); an option served saves one. The cap with the
+    largest saving, served - cost * candidates, wins (the smaller on ties); with none
+    positive the gate serves nothing: (-1, -1)."""
+    best = (-1.0, -1.0, 0.0)
     for c in caps:
         m = scale <= c
         if m.sum() < MIN_CALIBRATED:
             continue
         b = calibrate_spread(err[m], spread[m], bar=bar, cap=cap)
         n = int(np.sum(m & (spread <= b))) if b >= 0 else 0
-        if n > best[2]:
-            best = (b, float(c), n)
+        gain = n - cost * float(m.sum())
+        if n > 0 and gain > best[2]:
+            best = (b, float(c), gain)
     return best[0], best[1]
+
+
+def inference_cost_ratio(ctx: Context, sur: Surrogate, d_opts, n: int, fd) -> float:
+    """Time of one ensemble prediction per option over the time of one FD solve per
+    option, on the same n options and FD configuration (each timed after a warm-up
+    call): what running an option through the network costs, in FD solves."""
+    torch = _torch()
+    out = torch.empty(n, dtype=torch.float64, device="cuda")
+    st = torch.empty(n, dtype=torch.int32, device="cuda")
+
+    def timed(f):
+        f()
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        f()
+        ctx.synchronize()
+        return time.perf_counter() - t0
+
+    t_fd = timed(lambda: ctx.fd1d_price_device(d_opts.data_ptr(), n, out.data_ptr(),
+                                               st.data_ptr(), fd))
+    t_nn = timed(lambda: predict(ctx, sur, d_opts, n, fd))
+    return t_nn / t_fd
 
 
 def heldout_errors(ctx: Context, sur: Surrogate, opts, d_x=None, d_t=None):
@@ -192,13 +218,14 @@ def train_and_evaluate(ctx: Context, n_train=2_000_000, n_test=65536, members=4,
     cdims = W.serving_grids(cal, seed + 1100) if per_option_grids else (None, None)
     c_x, c_t = device_dims(len(cal), *cdims)
     e_c, s_c, c_c, _ = heldout_errors(ctx, sur, cal, c_x, c_t)
-    thr, scale_max = calibrate_gate(e_c, s_c, c_c)
+    kappa = inference_cost_ratio(ctx, sur, to_device(cal), len(cal), fd_config(c_x, c_t))
+    thr, scale_max = calibrate_gate(e_c, s_c, c_c, cost=kappa)
     served = (s_c <= thr) & (c_c <= scale_max)
     served_ho = (s_in <= thr) & (c_in <= scale_max)
     # the same calibration at looser error bars, for the accuracy / throughput trade-off
     looser = {}
     for bar in (1e-3, 1e-2):
-        tb, tc = calibrate_gate(e_c, s_c, c_c, bar=bar, cap=bar / 2)
+        tb, tc = calibrate_gate(e_c, s_c, c_c, bar=bar, cap=bar / 2, cost=kappa)
         sv = (s_c <= tb) & (c_c <= tc)
         looser[f"{bar:g}"] = {"spread_bound": tb, "scale_bound": tc,
                               "calibration_pass_fraction": float(np.mean(sv)),
@@ -212,6 +239,7 @@ def train_and_evaluate(ctx: Context, n_train=2_000_000, n_test=65536, members=4,
         "heldout_rms_abs_err": float(np.sqrt(np.mean(e_in * e_in))),
         "heldout_frac_within_bar": float(np.mean(np.abs(e_in) <= ERR_BAR)),
         "n_calibration": int(len(e_c)),
+        "inference_cost_ratio": kappa,
         "spread_bound": thr,
         "scale_bound": scale_max,
         "calibration_candidate_fraction": float(np.mean(c_c <= scale_max)),

@@ -68,3 +68,25 @@ def test_no_cap_with_enough_options_serves_nothing():
     spread = np.zeros(500)
     scale = np.zeros(500)
     assert pipeline.calibrate_gate(err, spread, scale) == (-1.0, -1.0)
+
+
+def test_inference_cost_picks_the_cap_that_saves_most():
+    """Each candidate (option under the scale cap) costs `cost` FD solves of inference and
+    each option served saves one. Here every cap serves its options with spread under
+    the bound, but the wider caps add candidates that are seldom served: a cost makes the
+    narrow cap win, and a cost above the served share turns serving off."""
+    rng = np.random.default_rng(4)
+    n = 200_000
+    scale = rng.uniform(0.0, 0.2, n)
+    # options above 0.02 have large spreads: candidates there are hardly ever served
+    spread = np.where(scale <= 0.02, rng.uniform(0.0, 1e-5, n), rng.uniform(0.0, 1e-3, n))
+    err = np.zeros(n)
+    thr0, cap0 = pipeline.calibrate_gate(err, spread, scale, bar=1e-4, cap=5e-5)
+    assert cap0 == float("inf")  # serving the most: no cap
+    thr1, cap1 = pipeline.calibrate_gate(err, spread, scale, bar=1e-4, cap=5e-5, cost=0.1)
+    assert cap1 == 0.02 and thr1 > 0
+    served = np.sum((scale <= cap1) & (spread <= thr1))
+    # the saving of the chosen cap is positive and beats the uncapped one's
+    assert served - 0.1 * np.sum(scale <= cap1) > (
+        np.sum((spread <= thr0)) - 0.1 * n)
+    assert pipeline.calibrate_gate(err, spread, scale, bar=1e-4, cap=5e-5, cost=2.0) == (-1.0, -1.0)

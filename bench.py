@@ -337,7 +337,7 @@ def bench_surrogate(ctx, args, log):
     uneven = {}
     if args.sur_per_option:
         sur2, train2 = pipeline.train_and_evaluate(ctx, n_train=args.sur_train - n_test,
-                                                   n_test=n_test, epochs=args.sur_epochs // 2,
+                                                   n_test=n_test, epochs=args.sur_epochs,
                                                    per_option_grids=True, log=log)
         xd, td = W.serving_grids(opts, 5)
         uneven["training"] = train2

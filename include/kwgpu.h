@@ -61,7 +61,9 @@ enum {
     KWGPU_STATUS_OK = 0,
     KWGPU_STATUS_OUT_OF_GRID = 1, /* Fd1d::value range error, src/Math/kwFd1d.cpp:151-153 */
     KWGPU_STATUS_BAD_INPUT = 2,   /* T<=0, sigma<=0, S<=0, K<=0 or non-finite field        */
-    KWGPU_STATUS_NONFINITE = 3    /* the solve produced a non-finite price                 */
+    KWGPU_STATUS_NONFINITE = 3,   /* the solve produced a non-finite price                 */
+    KWGPU_STATUS_PRECISION = 4    /* FP32-mixed only: the PDE is beyond the mixed march's
+                                     drift limit (see KWGPU_FP32_MIXED); solve it in FP64 */
 };
 
 /* Arithmetic of the FD march. */
@@ -70,12 +72,19 @@ enum {
     KWGPU_FP32_MIXED = 1 /* FP64 grid, coefficients, LU pivots, step residual, V
                             accumulation, projection and interpolation; FP32
                             substitutions, carries and fix-ups: <= 1e-4 vs FP64.
-                            With scale * sigma * sqrt(T) > 80 a call's FP32
-                            increments can overflow. The host entry points solve
-                            such PDEs in FP64; kwgpu_fd1d_price_device returns
-                            KWGPU_STATUS_NONFINITE for a call that overflows. Validated for sigma 0.02-1.2, T 0.002-15,
-                            r -0.03-0.2, q 0-0.25; at 1% vol with |r - q| T of ~3
-                            or more it can miss the bar without a status.            */
+                            Two limits, per PDE. Overflow: with scale * sigma *
+                            sqrt(T) > 80 a call's FP32 increments can overflow.
+                            Drift: with |r - q - sigma^2/2| T > scale * sigma *
+                            sqrt(T) the solution drifts past the mesh's half-width
+                            and the FP32 substitutions miss the bar (1 % vol over
+                            10-30 years at a 25-30 % yield: errors to 4e-3). The
+                            host entry points and the surrogate's FP32-mixed
+                            fallback solve such PDEs in FP64.
+                            kwgpu_fd1d_price_device (and _noalloc) solve as asked:
+                            an overflowing call returns KWGPU_STATUS_NONFINITE, a
+                            PDE past the drift limit KWGPU_STATUS_PRECISION (price
+                            NaN). Validated for sigma 0.02-1.2, T 0.002-15, r
+                            -0.03-0.2, q 0-0.25 within both limits.                  */
 };
 
 /* Where one option's system lives during the march. */

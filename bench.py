@@ -62,6 +62,12 @@ def parse_args():
                          "(FD FP64 1024x1024); 5 %% of them are held out")
     ap.add_argument("--sur-serve", type=int, default=1 << 20)
     ap.add_argument("--sur-epochs", type=int, default=320)
+    ap.add_argument("--sur-save", default="",
+                    help="save the fitted 1024 x 1024 surrogate to this file")
+    ap.add_argument("--sur-load", default="",
+                    help="load the 1024 x 1024 surrogate from this file (saved by --sur-save "
+                         "from this configuration) instead of fitting it; it is evaluated and "
+                         "its gate calibrated as after a fit")
     ap.add_argument("--sur-per-option", type=int, default=1,
                     help="also fit a surrogate to the per-option-grid FD pricer and serve "
                          "the batch on per-option grids")
@@ -348,8 +354,11 @@ def bench_surrogate(ctx, args, log, head=None):
     from kwgpu import pipeline
     from kwgpu import workloads as W
     n_test = args.sur_train // 20  # 5 % of the labelled set held out
+    fitted = kwgpu.Surrogate.load(ctx, args.sur_load) if args.sur_load else None
     sur, train = pipeline.train_and_evaluate(ctx, n_train=args.sur_train - n_test, n_test=n_test,
-                                             epochs=args.sur_epochs, log=log)
+                                             epochs=args.sur_epochs, fitted=fitted, log=log)
+    if args.sur_save:
+        sur.save(args.sur_save)
     opts = W.serving(args.sur_serve, 4)
     bound, smax = train["spread_bound"], train["scale_bound"]
     headline = None

@@ -122,7 +122,11 @@ def calibrate_spread(err, spread, bar=ERR_BAR, cap=SPREAD_MAX, margin=0.5,
 SCALE_CAPS = (0.01, 0.015, 0.02, 0.03, 0.05, 0.1, float("inf"))
 
 
-def calibrate_gate(err, spread, scale, bar=ERR_BAR, cap=SPREAD_MAX, caps=SCALE_CAPS, cost=0.0):
+MIN_GAIN = 0.01  # serve only when the gate saves at least this share of the FD solves
+
+
+def calibrate_gate(err, spread, scale, bar=ERR_BAR, cap=SPREAD_MAX, caps=SCALE_CAPS, cost=0.0,
+                   min_gain=0.0):
     """(spread bound, scale bound) of the gate: an option is served when its baseline
     scale (baseline premium + 1e-4 K, the price scale of the network's error) is at most
     the scale bound and its ensemble spread at most the spread bound. For each candidate
@@ -131,8 +135,10 @@ def calibrate_gate(err, spread, scale, bar=ERR_BAR, cap=SPREAD_MAX, caps=SCALE_C
     baseline premium is large, a small spread does not bound the error.
     Every option under the cap (a candidate) is run through the ensemble, which costs
     `cost` FD solves (inference_cost_ratio); an option served saves one. The cap with the
-    largest saving, served - cost * candidates, wins (the smaller on ties); with none
-    positive the gate serves nothing: (-1, -1)."""
+    largest saving, (served - cost * candidates) / held-out options, wins (the smaller on
+    ties). A saving below `min_gain` is not worth the error tail a batch may show beyond
+    the held-out maximum: then, as with no saving at all, the gate serves nothing:
+    (-1, -1)."""
     best = (-1.0, -1.0, 0.0)
     for c in caps:
         m = scale <= c
@@ -140,8 +146,8 @@ def calibrate_gate(err, spread, scale, bar=ERR_BAR, cap=SPREAD_MAX, caps=SCALE_C
             continue
         b = calibrate_spread(err[m], spread[m], bar=bar, cap=cap)
         n = int(np.sum(m & (spread <= b))) if b >= 0 else 0
-        gain = n - cost * float(m.sum())
-        if n > 0 and gain > best[2]:
+        gain = (n - cost * float(m.sum())) / len(err)
+        if n > 0 and gain > best[2] and gain >= min_gain:
             best = (b, float(c), gain)
     return best[0], best[1]
 
@@ -189,26 +195,37 @@ def heldout_errors(ctx: Context, sur: Surrogate, opts, d_x=None, d_t=None):
 
 
 def train_and_evaluate(ctx: Context, n_train=2_000_000, n_test=65536, members=4, epochs=160,
-                       batch=8192, seed=3, per_option_grids=False, n_calibrate=1 << 20,
-                       log=print):
+                       batch=8192, seed=3, per_option_grids=False, n_calibrate=4 << 20,
+                       fitted: Surrogate | None = None, log=print):
     """Fits the ensemble to the FD pricer, reports its error on the held-out options, and
     calibrates the gate on a separate set of `n_calibrate` options (seed + 1000) priced by
-    FD: as many as a serving batch, so that the bounds see the error tail a batch of that
-    size shows (a 100k-option calibration once admitted a 1.2e-4 error on a 1M batch)."""
+    FD. With k held-out options under the bounds, each new option under them exceeds
+    their largest error with probability <= 1/(k+1), so a batch serving as many options
+    as k expects one such option: the set is four times a 1M serving batch, so that one
+    is expected in four such batches, and the margin of half the bar absorbs it (a 100k
+    set once admitted a 1.2e-4 error on a 1M batch; a 1M set once admitted 1.8e-4 for
+    the per-option ensemble, which saved 0.4 % of the FD work).
+    `fitted`: a surrogate already fitted to this configuration (Surrogate.load), which is
+    evaluated and calibrated the same way without fitting again."""
     opts = W.surrogate_train(n_train + n_test, seed)
     dims = W.serving_grids(opts, seed + 100) if per_option_grids else (None, None)
     d_x, d_t = device_dims(len(opts), *dims)
-    d_o = to_device(opts)
-    sur = Surrogate(ctx, members=members, seed=seed)
-    fd_tr = fd_config(None if d_x is None else d_x[:n_train],
-                      None if d_t is None else d_t[:n_train])
-    t0 = time.perf_counter()
-    loss, pairs = sur.fit(d_o.data_ptr(), n_train, fd_tr, epochs=epochs, batch=batch)
-    t_fit = time.perf_counter() - t0
     grid = "per-option grids" if per_option_grids else f"{X_DIM}x{T_DIM}"
-    log(f"fit: {pairs} of {n_train} options labelled at their two bracketing nodes "
-        f"(FD FP64, {grid}), {members} members x {epochs} epochs in {t_fit:.2f} s, "
-        f"final loss {loss[-1].tolist()}")
+    if fitted is None:
+        d_o = to_device(opts)
+        sur = Surrogate(ctx, members=members, seed=seed)
+        fd_tr = fd_config(None if d_x is None else d_x[:n_train],
+                          None if d_t is None else d_t[:n_train])
+        t0 = time.perf_counter()
+        loss, pairs = sur.fit(d_o.data_ptr(), n_train, fd_tr, epochs=epochs, batch=batch)
+        t_fit = time.perf_counter() - t0
+        final_loss = loss[-1].tolist()
+        log(f"fit: {pairs} of {n_train} options labelled at their two bracketing nodes "
+            f"(FD FP64, {grid}), {members} members x {epochs} epochs in {t_fit:.2f} s, "
+            f"final loss {final_loss}")
+    else:
+        sur, pairs, t_fit, final_loss, epochs = fitted, 0, 0.0, None, 0
+        members = sur.members
     # held-out options at their own spots, against the FD price on the same grid
     ho = slice(n_train, n_train + n_test)
     e_in, s_in, c_in, n_in = heldout_errors(ctx, sur, opts[ho], None if d_x is None else d_x[ho],
@@ -219,13 +236,14 @@ def train_and_evaluate(ctx: Context, n_train=2_000_000, n_test=65536, members=4,
     c_x, c_t = device_dims(len(cal), *cdims)
     e_c, s_c, c_c, _ = heldout_errors(ctx, sur, cal, c_x, c_t)
     kappa = inference_cost_ratio(ctx, sur, to_device(cal), len(cal), fd_config(c_x, c_t))
-    thr, scale_max = calibrate_gate(e_c, s_c, c_c, cost=kappa)
+    thr, scale_max = calibrate_gate(e_c, s_c, c_c, cost=kappa, min_gain=MIN_GAIN)
     served = (s_c <= thr) & (c_c <= scale_max)
     served_ho = (s_in <= thr) & (c_in <= scale_max)
     # the same calibration at looser error bars, for the accuracy / throughput trade-off
     looser = {}
     for bar in (1e-3, 1e-2):
-        tb, tc = calibrate_gate(e_c, s_c, c_c, bar=bar, cap=bar / 2, cost=kappa)
+        tb, tc = calibrate_gate(e_c, s_c, c_c, bar=bar, cap=bar / 2, cost=kappa,
+                                min_gain=MIN_GAIN)
         sv = (s_c <= tb) & (c_c <= tc)
         looser[f"{bar:g}"] = {"spread_bound": tb, "scale_bound": tc,
                               "calibration_pass_fraction": float(np.mean(sv)),
@@ -234,7 +252,7 @@ def train_and_evaluate(ctx: Context, n_train=2_000_000, n_test=65536, members=4,
     out = {
         "grid": grid, "n_train": n_train, "n_labelled_pairs": int(pairs), "n_test": n_test,
         "n_test_in_box": n_in, "members": members, "epochs": epochs,
-        "fit_s": t_fit, "final_loss": loss[-1].tolist(),
+        "fit_s": t_fit, "final_loss": final_loss,
         "heldout_max_abs_err": float(np.max(np.abs(e_in))),
         "heldout_rms_abs_err": float(np.sqrt(np.mean(e_in * e_in))),
         "heldout_frac_within_bar": float(np.mean(np.abs(e_in) <= ERR_BAR)),

@@ -1,7 +1,8 @@
 """numpy restatement of the Barone-Adesi & Whaley (1987) early-exercise premium with a
 continuous dividend yield, used to check the device function the surrogate's target is
-taken relative to. Bisection for the critical spot as in the device code (64 halvings in
-log S over [K/1000, K] for puts, [K, 10^4 K] for calls)."""
+taken relative to. The critical spot by bisection (64 halvings in log S over [K/1000, K]
+for puts, [K, 10^4 K] for calls); the device solves the same equation over the same
+bracket by the Illinois method, so the two agree only if both find its root."""
 from __future__ import annotations
 
 import numpy as np

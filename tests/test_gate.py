@@ -90,3 +90,21 @@ def test_inference_cost_picks_the_cap_that_saves_most():
     assert served - 0.1 * np.sum(scale <= cap1) > (
         np.sum((spread <= thr0)) - 0.1 * n)
     assert pipeline.calibrate_gate(err, spread, scale, bar=1e-4, cap=5e-5, cost=2.0) == (-1.0, -1.0)
+
+
+def test_small_saving_serves_nothing():
+    """A gate that would save less than min_gain of the FD solves serves nothing: a small
+    served share is not worth the error tail a batch can show beyond the calibration."""
+    rng = np.random.default_rng(5)
+    n = 200_000
+    scale = rng.uniform(0.0, 0.2, n)
+    spread = rng.uniform(0.0, 1e-3, n)  # about 1 % of options under a 1e-5 spread bound
+    err = np.zeros(n)
+    thr, cap = pipeline.calibrate_gate(err, spread, scale, bar=1e-4, cap=1e-5)
+    assert thr > 0
+    served = np.mean((spread <= thr) & (scale <= cap))
+    assert 0.005 < served < 0.02
+    assert pipeline.calibrate_gate(err, spread, scale, bar=1e-4, cap=1e-5,
+                                   min_gain=2 * served) == (-1.0, -1.0)
+    assert pipeline.calibrate_gate(err, spread, scale, bar=1e-4, cap=1e-5,
+                                   min_gain=served / 2)[0] == thr

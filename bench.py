@@ -312,19 +312,22 @@ def bench_next_rows(ctx, opts, cpu_threads, log):
     return out
 
 
-def time_surrogate_first(ctx, sur, d_opts, n, bound, smax, steps, warmup, p64):
-    """The surrogate-first form on a batch already in HBM, timed like the FD forms:
+def time_surrogate_first(ctx, sur, d_opts, n, gate, steps, warmup, p64):
+    """The surrogate-first form under `gate` (a gate dict of kwgpu.pipeline) on a batch
+    already in HBM, timed like the FD forms:
     `warmup` untimed calls, then `steps` calls between two synchronizations (each call
     gates, runs the ensemble on the candidates, prices the rest by FP64 FD and scatters).
     Its prices are checked against the FP64 FD prices `p64` of the same options."""
     import torch
     import kwgpu
+    from kwgpu import pipeline
     d_p = torch.empty(n, dtype=torch.float64, device="cuda")
     d_r = torch.empty(n, dtype=torch.int32, device="cuda")
     fd = kwgpu.params(X_DIM, T_DIM)
+    g = pipeline.to_gate(gate)
 
     def call():
-        return sur.price(d_opts.data_ptr(), n, bound, fd, d_p.data_ptr(), d_r.data_ptr(), smax)
+        return sur.price_gated(d_opts.data_ptr(), n, g, fd, d_p.data_ptr(), d_r.data_ptr())
 
     for _ in range(warmup):
         call()
@@ -360,24 +363,27 @@ def bench_surrogate(ctx, args, log, head=None):
     if args.sur_save:
         sur.save(args.sur_save)
     opts = W.serving(args.sur_serve, 4)
-    bound, smax = train["spread_bound"], train["scale_bound"]
+    gate = train["gate"]
     headline = None
     if head is not None:
         d_h, n_h, p64_h, steps, warmup = head
-        headline = time_surrogate_first(ctx, sur, d_h, n_h, bound, smax, steps, warmup, p64_h)
+        headline = time_surrogate_first(ctx, sur, d_h, n_h, gate, steps, warmup, p64_h)
         log(f"bench: surrogate-first on the headline batch {headline['ms_per_step']:.3f} "
             f"ms/step, {headline['served_by_surrogate']} served, max err "
             f"{headline['max_abs_err_vs_fp64_fd']:.2e}")
-    serve = pipeline.serve(ctx, sur, opts, bound, log=log, scale_max=smax)
+    serve = pipeline.serve(ctx, sur, opts, gate, log=log)
     serve["meets_bar"] = serve["max_abs_err_vs_fp64_fd"] <= pipeline.ERR_BAR
-    serve32 = pipeline.serve(ctx, sur, opts, bound, log=log, precision=kwgpu.FP32_MIXED,
-                             scale_max=smax)
+    one = train["one_class_gate"]
+    serve1 = pipeline.serve(ctx, sur, opts, pipeline.one_class_gate(one["spread_bound"],
+                                                                     one["scale_bound"]),
+                            repeats=1, log=log)
+    serve1["meets_bar"] = serve1["max_abs_err_vs_fp64_fd"] <= pipeline.ERR_BAR
+    serve32 = pipeline.serve(ctx, sur, opts, gate, log=log, precision=kwgpu.FP32_MIXED)
     serve32["meets_bar"] = serve32["max_abs_err_vs_fp64_fd"] <= pipeline.ERR_BAR
     # the trade-off at looser bars (not the metric: its bar is 1e-4)
     looser = {}
     for bar, v in train["looser_bars"].items():
-        r = pipeline.serve(ctx, sur, opts, v["spread_bound"], repeats=1, log=log,
-                           scale_max=v["scale_bound"])
+        r = pipeline.serve(ctx, sur, opts, v["gate"], repeats=1, log=log)
         r["meets_bar"] = r["max_abs_err_vs_fp64_fd"] <= float(bar)
         looser[bar] = r
     sur.close()
@@ -389,14 +395,14 @@ def bench_surrogate(ctx, args, log, head=None):
         xd, td = W.serving_grids(opts, 5)
         uneven["training"] = train2
         for name, prec in (("fp64", kwgpu.FP64), ("fp32_mixed", kwgpu.FP32_MIXED)):
-            r = pipeline.serve(ctx, sur2, opts, train2["spread_bound"], log=log, x_dims=xd,
-                               t_dims=td, precision=prec, scale_max=train2["scale_bound"])
+            r = pipeline.serve(ctx, sur2, opts, train2["gate"], log=log, x_dims=xd, t_dims=td,
+                               precision=prec)
             r["meets_bar"] = r["max_abs_err_vs_fp64_fd"] <= pipeline.ERR_BAR
             uneven[name + "_fallback"] = r
         sur2.close()
     return {"training": train, "headline_batch": headline, "serving": serve,
-            "serving_fp32_mixed_fallback": serve32, "serving_per_option_grids": uneven,
-            "serving_at_looser_bars": looser}
+            "serving_one_class_gate": serve1, "serving_fp32_mixed_fallback": serve32,
+            "serving_per_option_grids": uneven, "serving_at_looser_bars": looser}
 
 
 def main():

@@ -317,8 +317,8 @@ int kwgpu_surrogate_predict(kwgpu_ctx* ctx, kwgpu_surrogate* s, const kwgpu_opti
  * The ensemble runs only on candidates: in-box options with a labelled bracket whose
  * baseline scale (baseline premium + 1e-4 K, the price scale of the network's error) is
  * at most scale_max (INFINITY: every in-box option); the others go to FD directly
- * (fd->precision FP64, or FP32_MIXED with options whose exp(scale sigma sqrt(T)) would
- * overflow FP32 sent to FP64). fd->x_dims / fd->t_dims, when set, are device arrays of
+ * (fd->precision FP64, or FP32_MIXED with the options beyond the mixed march's limits,
+ * see KWGPU_FP32_MIXED, sent to FP64). fd->x_dims / fd->t_dims, when set, are device arrays of
  * n per-option grid sizes, and each routed option is solved on its own. The surrogate
  * serves only when `fd` is the configuration it emulates (same density, scale, theta
  * and grid, or per-option grids for a per-option surrogate); otherwise, or with a
@@ -333,6 +333,36 @@ int kwgpu_surrogate_price(kwgpu_ctx* ctx, kwgpu_surrogate* s, const kwgpu_option
                           size_t n, double spread_max, double scale_max,
                           const kwgpu_fd1d_params* fd, double* d_prices, int32_t* d_route,
                           size_t* n_fd);
+
+/* The gate by option class. Classes: put/call x n_t maturity bands x n_m log-moneyness
+ * bands, split at the inner edges t_edges (T, years) and m_edges (log(S/K)), each
+ * increasing; a value equal to an edge falls in the band above it. Class (c, a, b) --
+ * c = 0 for puts (w < 0), 1 for calls, band a of T, band b of log(S/K) -- has index
+ * (c * n_t + a) * n_m + b. Its in-box options are served under spread_max[index] and
+ * scale_max[index] exactly as kwgpu_surrogate_price serves every option under its two
+ * bounds; a negative spread_max closes the class. Bounds calibrated on each class apart
+ * keep, class by class, what a calibration guarantees (a new option under the bounds
+ * exceeds the k held-out options' largest error with probability at most 1/(k+1)),
+ * while classes whose errors are small serve more. */
+#define KWGPU_GATE_MAX_T 8
+#define KWGPU_GATE_MAX_M 4
+#define KWGPU_GATE_MAX_CLASSES (2 * KWGPU_GATE_MAX_T * KWGPU_GATE_MAX_M)
+typedef struct kwgpu_gate {
+    int32_t n_t;                                /* maturity bands, 1..KWGPU_GATE_MAX_T   */
+    int32_t n_m;                                /* log-moneyness bands, 1..MAX_M         */
+    double t_edges[KWGPU_GATE_MAX_T - 1];       /* n_t - 1 used                          */
+    double m_edges[KWGPU_GATE_MAX_M - 1];       /* n_m - 1 used                          */
+    double spread_max[KWGPU_GATE_MAX_CLASSES];  /* 2 n_t n_m used                        */
+    double scale_max[KWGPU_GATE_MAX_CLASSES];
+} kwgpu_gate;
+
+/* kwgpu_surrogate_price with the bounds of `gate`'s classes (no reference counterpart).
+ * KWGPU_ERR_INVALID for band counts outside their ranges or edges that are not finite
+ * and increasing. With one band each (n_t = n_m = 1: two classes, puts and calls) and
+ * the same bounds for both classes it is kwgpu_surrogate_price. */
+int kwgpu_surrogate_price_gated(kwgpu_ctx* ctx, kwgpu_surrogate* s, const kwgpu_option* d_opts,
+                                size_t n, const kwgpu_gate* gate, const kwgpu_fd1d_params* fd,
+                                double* d_prices, int32_t* d_route, size_t* n_fd);
 
 /* Weights of one member as FP32 (KWGPU_SURROGATE_PARAMS values, host memory). */
 int kwgpu_surrogate_get_params(kwgpu_ctx* ctx, const kwgpu_surrogate* s, int32_t member,

@@ -758,8 +758,19 @@ __global__ void zero_f(float* p, size_t n) {
 }
 
 // ---- serving ---------------------------------------------------------------------
-// Route 0: surrogate (in the box, ensemble spread <= spread_max): price = BS + premium;
-// premium == nullptr (no prediction made) routes every option to FD.
+// The option's class in a gate: put/call, then its maturity band, then its band of
+// log(S/K) (kwgpu_gate in kwgpu.h).
+__device__ __forceinline__ int gate_class(const kwgpu_gate& g, const kwgpu_option& o) {
+    int a = 0;
+    while (a < g.n_t - 1 && o.t >= g.t_edges[a]) ++a;
+    const double m = log(o.s / o.k);
+    int b = 0;
+    while (b < g.n_m - 1 && m >= g.m_edges[b]) ++b;
+    return ((o.w < 0 ? 0 : 1) * g.n_t + a) * g.n_m + b;
+}
+
+// Route 0: surrogate (in the box, ensemble spread within its class's spread bound): price
+// = BS + premium; premium == nullptr (no prediction made) routes every option to FD.
 // Route 1: the FD fallback at its own precision. Route 2: FP64 FD, when the fallback is
 // FP32-mixed, for options beyond the mixed march's limits (mixed_overflow_risk,
 // mixed_drift_risk) at mesh scale mixed_scale (0: the fallback is FP64).
@@ -770,7 +781,7 @@ constexpr int kGateBlock = 256;
 
 __global__ __launch_bounds__(kGateBlock) void gate_kernel(
     const kwgpu_option* opts, size_t n, const double* premium, const double* spread,
-    const uint8_t* in_box, double spread_max, double mixed_scale, double* prices,
+    const uint8_t* in_box, const kwgpu_gate gate, double mixed_scale, double* prices,
     int32_t* route, uint32_t* blk_counts) {
     __shared__ uint32_t wc[2][kGateBlock / 64];
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -778,7 +789,8 @@ __global__ __launch_bounds__(kGateBlock) void gate_kernel(
     if (i < n) {
         const kwgpu_option o = opts[i];
         double price = NAN;
-        if (premium && in_box[i] && spread[i] <= spread_max && isfinite(premium[i])) {
+        if (premium && in_box[i] && spread[i] <= gate.spread_max[gate_class(gate, o)] &&
+            isfinite(premium[i])) {
             kwgpu_option e = o;
             e.e = 0;
             price = bs_one(e) + premium[i];
@@ -870,12 +882,13 @@ __global__ __launch_bounds__(kGateBlock) void compact_routes(const int32_t* rout
     }
 }
 
-// Serving candidates: in the training box, with a labelled bracket, and a baseline scale
-// baw + 1e-4 K (the price scale of the network's error) of at most scale_max; flag[i] = 1
-// for a candidate, else 0, with per-block counts as gate_kernel keeps them (so that
-// compact_routes lists the candidates in index order).
+// Serving candidates: in the training box, with a labelled bracket, in a class the gate
+// leaves open, and with a baseline scale baw + 1e-4 K (the price scale of the network's
+// error) of at most that class's scale bound; flag[i] = 1 for a candidate, else 0, with
+// per-block counts as gate_kernel keeps them (so that compact_routes lists the
+// candidates in index order).
 __global__ __launch_bounds__(kGateBlock) void candidate_kernel(const kwgpu_option* opts, size_t n,
-                                                               GridArg g, double scale_max,
+                                                               GridArg g, const kwgpu_gate gate,
                                                                int32_t* flag, uint32_t* blk_counts) {
     __shared__ uint32_t wc[kGateBlock / 64];
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -884,8 +897,9 @@ __global__ __launch_bounds__(kGateBlock) void candidate_kernel(const kwgpu_optio
         const kwgpu_option o = opts[i];
         float f[F];
         const bool in = option_features(o, g.x(i), g.t(i), f);
-        if (in && node_pair(o, g.x(i), g.t(i), g.density, g.scale).ok)
-            c = baw_premium(o) + kEps * o.k <= scale_max ? 1 : 0;
+        const int k = gate_class(gate, o);
+        if (in && gate.spread_max[k] >= 0.0 && node_pair(o, g.x(i), g.t(i), g.density, g.scale).ok)
+            c = baw_premium(o) + kEps * o.k <= gate.scale_max[k] ? 1 : 0;
         flag[i] = c;
     }
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -1629,10 +1643,56 @@ int kwgpu_surrogate_predict(kwgpu_ctx* ctx, kwgpu_surrogate* s, const kwgpu_opti
     return KWGPU_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+bool gate_valid(const kwgpu_gate& g) {
+    if (g.n_t < 1 || g.n_t > KWGPU_GATE_MAX_T || g.n_m < 1 || g.n_m > KWGPU_GATE_MAX_M)
+        return false;
+    for (int a = 0; a + 1 < g.n_t; ++a)
+        if (!std::isfinite(g.t_edges[a]) || (a > 0 && !(g.t_edges[a] > g.t_edges[a - 1])))
+            return false;
+    for (int b = 0; b + 1 < g.n_m; ++b)
+        if (!std::isfinite(g.m_edges[b]) || (b > 0 && !(g.m_edges[b] > g.m_edges[b - 1])))
+            return false;
+    return true;
+}
+
+int price_gated(kwgpu_ctx* ctx, kwgpu_surrogate* s, const kwgpu_option* d_opts, size_t n,
+                const kwgpu_gate& gate, const kwgpu_fd1d_params* fd, double* d_prices,
+                int32_t* d_route, size_t* n_fd);
+
+}  // namespace
+
+extern "C" {
+
 int kwgpu_surrogate_price(kwgpu_ctx* ctx, kwgpu_surrogate* s, const kwgpu_option* d_opts,
                           size_t n, double spread_max, double scale_max,
                           const kwgpu_fd1d_params* fd, double* d_prices, int32_t* d_route,
                           size_t* n_fd) {
+    kwgpu_gate g{};  // one band each: the two classes are puts and calls
+    g.n_t = g.n_m = 1;
+    g.spread_max[0] = g.spread_max[1] = spread_max;
+    g.scale_max[0] = g.scale_max[1] = scale_max;
+    return price_gated(ctx, s, d_opts, n, g, fd, d_prices, d_route, n_fd);
+}
+
+int kwgpu_surrogate_price_gated(kwgpu_ctx* ctx, kwgpu_surrogate* s, const kwgpu_option* d_opts,
+                                size_t n, const kwgpu_gate* gate, const kwgpu_fd1d_params* fd,
+                                double* d_prices, int32_t* d_route, size_t* n_fd) {
+    if (!gate || !gate_valid(*gate))
+        return kwgpu::ctx_fail(ctx, KWGPU_ERR_INVALID, "kwgpu_surrogate_price_gated: bad gate");
+    return price_gated(ctx, s, d_opts, n, *gate, fd, d_prices, d_route, n_fd);
+}
+
+}  // extern "C"
+
+namespace {
+
+int price_gated(kwgpu_ctx* ctx, kwgpu_surrogate* s, const kwgpu_option* d_opts, size_t n,
+                const kwgpu_gate& gate, const kwgpu_fd1d_params* fd, double* d_prices,
+                int32_t* d_route, size_t* n_fd) {
     if (!s || !d_opts || !d_prices || !d_route || !fd)
         return kwgpu::ctx_fail(ctx, KWGPU_ERR_INVALID, "kwgpu_surrogate_price: bad argument");
     if (n_fd) *n_fd = 0;
@@ -1679,9 +1739,12 @@ int kwgpu_surrogate_price(kwgpu_ctx* ctx, kwgpu_surrogate* s, const kwgpu_option
     int* cnt = (int*)(base + at);
     int rc = KWGPU_OK;
     auto cleanup = [&]() { (void)hipStreamSynchronize(st); };
-    // a negative (or NaN) bound passes no option, and neither does an FD configuration
-    // other than the one the surrogate emulates: the ensemble is not evaluated
-    const bool serve_none = !(spread_max >= 0.0) || !grid_matches(s, fd);
+    // a negative (or NaN) bound passes no option of its class, and an FD configuration
+    // other than the one the surrogate emulates passes none at all: with every class
+    // closed the ensemble is not evaluated
+    bool open = false;
+    for (int k = 0; k < 2 * gate.n_t * gate.n_m; ++k) open = open || gate.spread_max[k] >= 0.0;
+    const bool serve_none = !open || !grid_matches(s, fd);
     const unsigned nb = nblk(n, kGateBlock);
     if (serve_none && fd->precision == KWGPU_FP64) {
         // nothing to route: the batch goes to the FD pricer as it is, with no compaction
@@ -1700,7 +1763,7 @@ int kwgpu_surrogate_price(kwgpu_ctx* ctx, kwgpu_surrogate* s, const kwgpu_option
         // their grid sizes, predicted, and their predictions scattered back
         const GridArg g = grid_arg(s, fd);
         hipLaunchKernelGGL(candidate_kernel, dim3(nb), dim3(kGateBlock), 0, st, d_opts, n, g,
-                           scale_max, d_route, blk);
+                           gate, d_route, blk);
         hipLaunchKernelGGL(scan_block_counts, dim3(1), dim3(1024), 0, st, blk, nb, cnt);
         hipLaunchKernelGGL(compact_routes, dim3(nb), dim3(kGateBlock), 0, st, d_route, n, blk,
                            (const int*)cnt, idx);
@@ -1735,7 +1798,7 @@ int kwgpu_surrogate_price(kwgpu_ctx* ctx, kwgpu_surrogate* s, const kwgpu_option
     // an FP32-mixed fallback sends the options beyond its limits to FP64
     const double mixed_scale = fd->precision == KWGPU_FP64 ? 0.0 : fd->scale;
     hipLaunchKernelGGL(gate_kernel, dim3(nb), dim3(kGateBlock), 0, st, d_opts, n,
-                       serve_none ? nullptr : prem, spr, inb, spread_max, mixed_scale, d_prices,
+                       serve_none ? nullptr : prem, spr, inb, gate, mixed_scale, d_prices,
                        d_route, blk);
     hipLaunchKernelGGL(scan_block_counts, dim3(1), dim3(1024), 0, st, blk, nb, cnt);
     hipLaunchKernelGGL(compact_routes, dim3(nb), dim3(kGateBlock), 0, st, d_route, n, blk,
@@ -1781,6 +1844,10 @@ int kwgpu_surrogate_price(kwgpu_ctx* ctx, kwgpu_surrogate* s, const kwgpu_option
     cleanup();
     return rc;
 }
+
+}  // namespace
+
+extern "C" {
 
 int kwgpu_surrogate_save(kwgpu_ctx* ctx, const kwgpu_surrogate* s, const char* path) {
     if (!s || !path) return kwgpu::ctx_fail(ctx, KWGPU_ERR_INVALID, "kwgpu_surrogate_save: bad argument");

@@ -30,7 +30,7 @@ EXPORTED_SYMBOLS = (
     "kwgpu_surrogate_train_params_default", "kwgpu_surrogate_create", "kwgpu_surrogate_destroy",
     "kwgpu_surrogate_train", "kwgpu_surrogate_labels", "kwgpu_surrogate_fit",
     "kwgpu_surrogate_predict",
-    "kwgpu_surrogate_price",
+    "kwgpu_surrogate_price", "kwgpu_surrogate_price_gated",
     "kwgpu_surrogate_get_params", "kwgpu_surrogate_set_params", "kwgpu_surrogate_loss_grad",
     "kwgpu_surrogate_features", "kwgpu_surrogate_baseline", "kwgpu_surrogate_save", "kwgpu_surrogate_load",
     "kwgpu_fd1d_autotune", "kwgpu_tune_save", "kwgpu_tune_load",
@@ -45,6 +45,47 @@ class SurrogateTrainParams(ctypes.Structure):
         ("lr_final", ctypes.c_float), ("beta1", ctypes.c_float), ("beta2", ctypes.c_float),
         ("eps", ctypes.c_float), ("seed", ctypes.c_uint64),
     ]
+
+
+GATE_MAX_T, GATE_MAX_M = 8, 4
+GATE_MAX_CLASSES = 2 * GATE_MAX_T * GATE_MAX_M
+
+
+class Gate(ctypes.Structure):
+    """kwgpu_gate: per-class bounds over put/call x maturity bands x log-moneyness bands."""
+    _fields_ = [
+        ("n_t", ctypes.c_int32), ("n_m", ctypes.c_int32),
+        ("t_edges", ctypes.c_double * (GATE_MAX_T - 1)),
+        ("m_edges", ctypes.c_double * (GATE_MAX_M - 1)),
+        ("spread_max", ctypes.c_double * GATE_MAX_CLASSES),
+        ("scale_max", ctypes.c_double * GATE_MAX_CLASSES),
+    ]
+
+
+def gate(t_edges, m_edges, spread_max, scale_max) -> Gate:
+    """A Gate from its inner band edges and per-class bounds, indexed
+    (c * n_t + a) * n_m + b for c = 0 (puts) / 1 (calls), T band a, log(S/K) band b."""
+    g = Gate()
+    g.n_t, g.n_m = len(t_edges) + 1, len(m_edges) + 1
+    k = 2 * g.n_t * g.n_m
+    if g.n_t > GATE_MAX_T or g.n_m > GATE_MAX_M or len(spread_max) != k or len(scale_max) != k:
+        raise ValueError("gate: band counts or bound arrays do not fit")
+    for i, v in enumerate(t_edges):
+        g.t_edges[i] = v
+    for i, v in enumerate(m_edges):
+        g.m_edges[i] = v
+    for i in range(k):
+        g.spread_max[i], g.scale_max[i] = spread_max[i], scale_max[i]
+    return g
+
+
+def gate_class(opts, t_edges, m_edges) -> np.ndarray:
+    """Class index of each option under a gate with these band edges (as the device
+    assigns it: a value equal to an edge falls in the band above)."""
+    c = np.where(np.asarray(opts["w"]) < 0, 0, 1)
+    a = np.searchsorted(np.asarray(t_edges, float), opts["t"], side="right")
+    b = np.searchsorted(np.asarray(m_edges, float), np.log(opts["s"] / opts["k"]), side="right")
+    return (c * (len(t_edges) + 1) + a) * (len(m_edges) + 1) + b
 
 
 class Fd1dParams(ctypes.Structure):
@@ -105,6 +146,9 @@ def lib():
         L.kwgpu_surrogate_price.argtypes = [P, P, P, SZ, ctypes.c_double, ctypes.c_double,
                                             ctypes.POINTER(Fd1dParams), P, P,
                                             ctypes.POINTER(ctypes.c_size_t)]
+        L.kwgpu_surrogate_price_gated.argtypes = [P, P, P, SZ, ctypes.POINTER(Gate),
+                                                  ctypes.POINTER(Fd1dParams), P, P,
+                                                  ctypes.POINTER(ctypes.c_size_t)]
         L.kwgpu_surrogate_get_params.argtypes = [P, P, ctypes.c_int32, P]
         L.kwgpu_surrogate_set_params.argtypes = [P, P, ctypes.c_int32, P]
         L.kwgpu_surrogate_loss_grad.argtypes = [P, P, ctypes.c_int32, P, P, SZ, P, P]
@@ -374,6 +418,15 @@ class Surrogate:
         self._check(lib().kwgpu_surrogate_price(self.ctx.h, self.h, d_opts, n, spread_max,
                                                 scale_max, ctypes.byref(fd), d_prices, d_route,
                                                 ctypes.byref(nfd)))
+        return nfd.value
+
+    def price_gated(self, d_opts: int, n: int, g: Gate, fd: Fd1dParams, d_prices: int,
+                    d_route: int) -> int:
+        """price() with per-class bounds (kwgpu_gate); returns the number priced by FD."""
+        nfd = ctypes.c_size_t(0)
+        self._check(lib().kwgpu_surrogate_price_gated(self.ctx.h, self.h, d_opts, n,
+                                                      ctypes.byref(g), ctypes.byref(fd), d_prices,
+                                                      d_route, ctypes.byref(nfd)))
         return nfd.value
 
     def get_params(self, member: int) -> np.ndarray:

@@ -21,7 +21,8 @@ import time
 
 import numpy as np
 
-from . import FP64, Context, Surrogate, params, surrogate_baseline
+from . import FP64, Context, Gate, Surrogate, gate_class, params, surrogate_baseline
+from . import gate as make_gate
 from . import workloads as W
 
 X_DIM = 1024
@@ -139,17 +140,77 @@ def calibrate_gate(err, spread, scale, bar=ERR_BAR, cap=SPREAD_MAX, caps=SCALE_C
     ties). A saving below `min_gain` is not worth the error tail a batch may show beyond
     the held-out maximum: then, as with no saving at all, the gate serves nothing:
     (-1, -1)."""
+    b, c, gain = _best_cap(err, spread, scale, len(err), bar, cap, caps, cost, MIN_CALIBRATED)
+    return (b, c) if gain >= min_gain and b >= 0 else (-1.0, -1.0)
+
+
+def _best_cap(err, spread, scale, n_all, bar, cap, caps, cost, min_count):
+    """(spread bound, scale cap, saving as a share of n_all) of the cap that saves most,
+    or (-1, -1, 0) when none saves anything."""
     best = (-1.0, -1.0, 0.0)
     for c in caps:
         m = scale <= c
-        if m.sum() < MIN_CALIBRATED:
+        if m.sum() < min_count:
             continue
-        b = calibrate_spread(err[m], spread[m], bar=bar, cap=cap)
+        b = calibrate_spread(err[m], spread[m], bar=bar, cap=cap, min_count=min_count)
         n = int(np.sum(m & (spread <= b))) if b >= 0 else 0
-        gain = (n - cost * float(m.sum())) / len(err)
-        if n > 0 and gain > best[2] and gain >= min_gain:
+        gain = (n - cost * float(m.sum())) / n_all
+        if n > 0 and gain > best[2]:
             best = (b, float(c), gain)
-    return best[0], best[1]
+    return best
+
+
+# Classes of the gate: put/call x maturity bands x log-moneyness bands (kwgpu_gate)
+GATE_T_EDGES = (0.5, 2.0)
+GATE_M_EDGES = (0.0,)
+MIN_CALIBRATED_CLASS = 10000
+
+
+def n_classes(t_edges=GATE_T_EDGES, m_edges=GATE_M_EDGES) -> int:
+    return 2 * (len(t_edges) + 1) * (len(m_edges) + 1)
+
+
+def calibrate_classes(err, spread, scale, cls, t_edges=GATE_T_EDGES, m_edges=GATE_M_EDGES,
+                      bar=ERR_BAR, cap=SPREAD_MAX, caps=SCALE_CAPS, cost=0.0, min_gain=0.0,
+                      min_count=MIN_CALIBRATED_CLASS):
+    """The gate by class (kwgpu_gate): each class's (spread bound, scale bound) calibrated
+    as calibrate_gate calibrates the one gate, on that class's held-out options alone
+    (`cls`: their class indices, kwgpu.gate_class), with at least `min_count` of them
+    under its bounds. Each class then keeps the guarantee of its own calibration, and a
+    class whose errors are small is not held to the bound of one whose errors are large.
+    A class that saves nothing is closed (-1); if all classes together save less than
+    `min_gain` of the held-out options' FD solves, all are. Returns a dict of the band
+    edges and the per-class bounds (gate(), serve())."""
+    k = n_classes(t_edges, m_edges)
+    smax, cmax, total = [-1.0] * k, [-1.0] * k, 0.0
+    for c in range(k):
+        m = cls == c
+        if m.sum() < min_count:
+            continue
+        b, sc, gain = _best_cap(err[m], spread[m], scale[m], len(err), bar, cap, caps, cost,
+                                min_count)
+        if b >= 0:
+            smax[c], cmax[c], total = b, sc, total + gain
+    if total < min_gain:
+        smax, cmax = [-1.0] * k, [-1.0] * k
+    return {"t_edges": list(t_edges), "m_edges": list(m_edges), "spread_max": smax,
+            "scale_max": cmax}
+
+
+def one_class_gate(spread_max: float, scale_max: float = float("inf")):
+    """The one-class gate (kwgpu_surrogate_price's two bounds) as a gate dict."""
+    return {"t_edges": [], "m_edges": [], "spread_max": [spread_max] * 2,
+            "scale_max": [scale_max] * 2}
+
+
+def gate_mask(gate, opts, spread, scale):
+    """Which options a gate dict serves, given their spreads and scales (host)."""
+    c = gate_class(opts, gate["t_edges"], gate["m_edges"])
+    return (spread <= np.asarray(gate["spread_max"])[c]) & (scale <= np.asarray(gate["scale_max"])[c])
+
+
+def to_gate(g) -> Gate:
+    return make_gate(g["t_edges"], g["m_edges"], g["spread_max"], g["scale_max"])
 
 
 def inference_cost_ratio(ctx: Context, sur: Surrogate, d_opts, n: int, fd) -> float:
@@ -175,9 +236,9 @@ def inference_cost_ratio(ctx: Context, sur: Surrogate, d_opts, n: int, fd) -> fl
 
 
 def heldout_errors(ctx: Context, sur: Surrogate, opts, d_x=None, d_t=None):
-    """(error, spread, scale) of the surrogate's prices against FD at the options' own
-    spots (same configuration), for the in-box options with a labelled bracket whose FD
-    price succeeded; scale = baseline premium + 1e-4 K."""
+    """(error, spread, scale, count, options) of the surrogate's prices against FD at the
+    options' own spots (same configuration), for the in-box options with a labelled
+    bracket whose FD price succeeded; scale = baseline premium + 1e-4 K."""
     torch = _torch()
     n = len(opts)
     d_o = to_device(opts)
@@ -191,7 +252,7 @@ def heldout_errors(ctx: Context, sur: Surrogate, opts, d_x=None, d_t=None):
     inb = (ib.cpu().numpy() == 1) & (st == 0)
     err = (bs + pm.cpu().numpy()) - truth
     scale = surrogate_baseline(ctx, opts) + 1e-4 * opts["k"]
-    return err[inb], sp.cpu().numpy()[inb], scale[inb], int(inb.sum())
+    return err[inb], sp.cpu().numpy()[inb], scale[inb], int(inb.sum()), opts[inb]
 
 
 def train_and_evaluate(ctx: Context, n_train=2_000_000, n_test=65536, members=4, epochs=160,
@@ -228,25 +289,32 @@ def train_and_evaluate(ctx: Context, n_train=2_000_000, n_test=65536, members=4,
         members = sur.members
     # held-out options at their own spots, against the FD price on the same grid
     ho = slice(n_train, n_train + n_test)
-    e_in, s_in, c_in, n_in = heldout_errors(ctx, sur, opts[ho], None if d_x is None else d_x[ho],
-                                            None if d_t is None else d_t[ho])
+    e_in, s_in, c_in, n_in, o_in = heldout_errors(ctx, sur, opts[ho],
+                                                  None if d_x is None else d_x[ho],
+                                                  None if d_t is None else d_t[ho])
     # the gate, calibrated on its own set
     cal = W.surrogate_train(n_calibrate, seed + 1000)
     cdims = W.serving_grids(cal, seed + 1100) if per_option_grids else (None, None)
     c_x, c_t = device_dims(len(cal), *cdims)
-    e_c, s_c, c_c, _ = heldout_errors(ctx, sur, cal, c_x, c_t)
+    e_c, s_c, c_c, _, o_c = heldout_errors(ctx, sur, cal, c_x, c_t)
     kappa = inference_cost_ratio(ctx, sur, to_device(cal), len(cal), fd_config(c_x, c_t))
+    # the one gate (reported for comparison) and the gate by class (served with)
     thr, scale_max = calibrate_gate(e_c, s_c, c_c, cost=kappa, min_gain=MIN_GAIN)
-    served = (s_c <= thr) & (c_c <= scale_max)
-    served_ho = (s_in <= thr) & (c_in <= scale_max)
-    # the same calibration at looser error bars, for the accuracy / throughput trade-off
+    one_served = (s_c <= thr) & (c_c <= scale_max)
+    cls_c = gate_class(o_c, GATE_T_EDGES, GATE_M_EDGES)
+    gate = calibrate_classes(e_c, s_c, c_c, cls_c, cost=kappa, min_gain=MIN_GAIN)
+    served = gate_mask(gate, o_c, s_c, c_c)
+    served_ho = gate_mask(gate, o_in, s_in, c_in)
+    # the one gate at looser error bars, for the accuracy / throughput trade-off (by
+    # class, the 1e-2 bar once admitted a 1.07e-2 error on the 1M batch: with a spread cap
+    # of half such a bar the classes' tails are far longer than at 1e-4)
     looser = {}
     for bar in (1e-3, 1e-2):
         tb, tc = calibrate_gate(e_c, s_c, c_c, bar=bar, cap=bar / 2, cost=kappa,
                                 min_gain=MIN_GAIN)
-        sv = (s_c <= tb) & (c_c <= tc)
-        looser[f"{bar:g}"] = {"spread_bound": tb, "scale_bound": tc,
-                              "calibration_pass_fraction": float(np.mean(sv)),
+        gb = one_class_gate(tb, tc)
+        sv = gate_mask(gb, o_c, s_c, c_c)
+        looser[f"{bar:g}"] = {"gate": gb, "calibration_pass_fraction": float(np.mean(sv)),
                               "calibration_max_err_served":
                                   float(np.max(np.abs(e_c[sv]))) if sv.any() else 0.0}
     out = {
@@ -258,9 +326,11 @@ def train_and_evaluate(ctx: Context, n_train=2_000_000, n_test=65536, members=4,
         "heldout_frac_within_bar": float(np.mean(np.abs(e_in) <= ERR_BAR)),
         "n_calibration": int(len(e_c)),
         "inference_cost_ratio": kappa,
-        "spread_bound": thr,
-        "scale_bound": scale_max,
-        "calibration_candidate_fraction": float(np.mean(c_c <= scale_max)),
+        "gate": gate,
+        "one_class_gate": {"spread_bound": thr, "scale_bound": scale_max,
+                           "calibration_pass_fraction": float(np.mean(one_served))},
+        "calibration_candidate_fraction": float(np.mean(
+            c_c <= np.asarray(gate["scale_max"])[cls_c])),
         "calibration_pass_fraction": float(np.mean(served)),
         "calibration_max_err_served": float(np.max(np.abs(e_c[served]))) if served.any() else 0.0,
         "heldout_pass_fraction": float(np.mean(served_ho)),
@@ -271,9 +341,10 @@ def train_and_evaluate(ctx: Context, n_train=2_000_000, n_test=65536, members=4,
     return sur, out
 
 
-def serve(ctx: Context, sur: Surrogate, opts, spread_bound: float, repeats=3, log=print,
-          x_dims=None, t_dims=None, precision=FP64, scale_max=float("inf")):
-    """Surrogate-first pricing of `opts` against plain FP64 FD of the same batch:
+def serve(ctx: Context, sur: Surrogate, opts, gate, repeats=3, log=print,
+          x_dims=None, t_dims=None, precision=FP64):
+    """Surrogate-first pricing of `opts` under `gate` (a gate dict: calibrate_classes,
+    one_class_gate) against plain FP64 FD of the same batch:
     options/s of both, max/RMS error of the served prices vs FP64 FD. `x_dims`/`t_dims`
     (host int32 arrays, optional) give every option its own grid, for the FD fallback and
     for the FP64 FD reference alike. `precision` is the fallback's; FP64 is the default since
@@ -293,15 +364,14 @@ def serve(ctx: Context, sur: Surrogate, opts, spread_bound: float, repeats=3, lo
         ctx.fd1d_price_device(d_o.data_ptr(), n, d_ref.data_ptr(), d_st.data_ptr(), fd64)
         ctx.synchronize()
 
-    sur.price(d_o.data_ptr(), n, spread_bound, fd, d_p.data_ptr(), d_r.data_ptr(),
-              scale_max)  # warm-up
+    g = to_gate(gate)
+    sur.price_gated(d_o.data_ptr(), n, g, fd, d_p.data_ptr(), d_r.data_ptr())  # warm-up
     torch.cuda.synchronize()
     ts = []
     nfd = 0
     for _ in range(repeats):
         t0 = time.perf_counter()
-        nfd = sur.price(d_o.data_ptr(), n, spread_bound, fd, d_p.data_ptr(), d_r.data_ptr(),
-                        scale_max)
+        nfd = sur.price_gated(d_o.data_ptr(), n, g, fd, d_p.data_ptr(), d_r.data_ptr())
         torch.cuda.synchronize()
         ts.append(time.perf_counter() - t0)
     t_sur = min(ts)

@@ -322,7 +322,38 @@ int main(int argc, char** argv) {
                             expect(kwgpu_surrogate_price(c, sx, o.data(), m, bound, 0.05, &p,
                                                          pr.data(), route.data(), &nfd),
                                    KWGPU_OK, c, "surrogate price");
+                        // the same by class: 2 x 3 x 2 classes, every other one closed
+                        kwgpu_gate gt{};
+                        gt.n_t = 3;
+                        gt.n_m = 2;
+                        gt.t_edges[0] = 0.5;
+                        gt.t_edges[1] = 2.0;
+                        gt.m_edges[0] = 0.0;
+                        for (int k = 0; k < 12; ++k) {
+                            gt.spread_max[k] = k % 2 ? bound : -1.0;
+                            gt.scale_max[k] = 0.05;
+                        }
+                        expect(kwgpu_surrogate_price_gated(c, sx, o.data(), n, &gt, &p,
+                                                           pr.data(), route.data(), &nfd),
+                               KWGPU_OK, c, "surrogate price by class");
                     }
+        {
+            kwgpu_gate bad{};
+            bad.n_t = 2;
+            bad.n_m = 1;
+            bad.t_edges[0] = NAN;
+            expect(kwgpu_surrogate_price_gated(c, s, o.data(), n, &bad, &p, pr.data(), route.data(),
+                                               &nfd),
+                   KWGPU_ERR_INVALID, c, "gate with a NaN edge");
+            bad.t_edges[0] = 1.0;
+            bad.n_m = KWGPU_GATE_MAX_M + 1;
+            expect(kwgpu_surrogate_price_gated(c, s, o.data(), n, &bad, &p, pr.data(), route.data(),
+                                               &nfd),
+                   KWGPU_ERR_INVALID, c, "gate with too many bands");
+            expect(kwgpu_surrogate_price_gated(c, s, o.data(), n, nullptr, &p, pr.data(),
+                                               route.data(), &nfd),
+                   KWGPU_ERR_INVALID, c, "no gate");
+        }
         kwgpu_surrogate_destroy(sg);
         const std::string path = dir + "/sur.bin";
         expect(kwgpu_surrogate_save(c, s, path.c_str()), KWGPU_OK, c, "surrogate save");

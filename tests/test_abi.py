@@ -8,6 +8,7 @@ import os
 import re
 import subprocess
 
+import numpy as np
 import pytest
 
 import kwgpu
@@ -81,3 +82,29 @@ def test_cpp_api_headers_compile(tmp_path):
     src.write_text('#include "kw/kwPricerFactory.h"\n#include "kw/kwPortfolio.h"\nint main(){}\n')
     subprocess.check_call(["g++", "-std=c++20", "-fsyntax-only", "-I", os.path.join(ROOT, "include"),
                            str(src)])
+
+
+def test_gate_layout_matches_header(tmp_path):
+    """The ctypes Gate has kwgpu_gate's size and offsets, and gate_class assigns classes
+    in the header's order: (c * n_t + a) * n_m + b, an edge value in the band above."""
+    g = kwgpu.Gate
+    offs = {f: getattr(g, f).offset for f, _ in g._fields_}
+    src = tmp_path / "gate.cpp"
+    src.write_text(
+        '#include <cstddef>\n#include "kwgpu.h"\n'
+        f"static_assert(sizeof(kwgpu_gate) == {ctypes.sizeof(g)});\n" +
+        "".join(f"static_assert(offsetof(kwgpu_gate, {f}) == {o});\n" for f, o in offs.items()) +
+        "int main() { return 0; }\n")
+    subprocess.check_call(["g++", "-std=c++17", "-fsyntax-only", "-I", os.path.join(ROOT, "include"),
+                           str(src)])
+    o = np.zeros(4, dtype=kwgpu.OPTION_DTYPE)
+    o["k"] = 100.0
+    o["s"] = [100.0, 90.0, 110.0, 100.0]
+    o["t"] = [0.5, 0.1, 3.0, 1.0]
+    o["w"] = [-1, 1, 1, -1]
+    # T bands [0, 0.5), [0.5, 2), [2, inf); log(S/K) bands [<0), [>=0)
+    got = kwgpu.gate_class(o, [0.5, 2.0], [0.0])
+    assert got.tolist() == [(0 * 3 + 1) * 2 + 1, (1 * 3 + 0) * 2 + 0, (1 * 3 + 2) * 2 + 1,
+                            (0 * 3 + 1) * 2 + 1]
+    with pytest.raises(ValueError):
+        kwgpu.gate([0.5] * 8, [], [0.0] * 18, [0.0] * 18)

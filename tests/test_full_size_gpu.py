@@ -156,16 +156,21 @@ def test_training_labels_match_oracle(ctx):
 def test_training_heldout_error(fitted):
     sur, out = fitted
     _say("  held-out rms", out["heldout_rms_abs_err"], "max", out["heldout_max_abs_err"],
-         "pass", out["heldout_pass_fraction"], "bound", out["spread_bound"])
+         "pass", out["heldout_pass_fraction"], "gate", out["gate"])
     assert out["n_labelled_pairs"] >= out["n_train"] - 100
     assert all(np.isfinite(out["final_loss"]))
     assert out["heldout_rms_abs_err"] <= 3e-3
     assert out["heldout_frac_within_bar"] >= 0.3
-    # the gate, calibrated on its own 1M-option set, serves a share of it within half the
-    # bar, and of the held-out set within the bar
-    assert out["n_calibration"] > 1_000_000
-    assert 0 < out["spread_bound"] <= P.SPREAD_MAX and out["scale_bound"] > 0
+    # the gate by class, calibrated on its own 4M-option set, serves a share of it within
+    # half the bar, and of the held-out set within the bar; every open class's spread bound
+    # is within the cap, and the classes serve more than the one gate does
+    assert out["n_calibration"] > 4_000_000
+    g = out["gate"]
+    assert len(g["spread_max"]) == P.n_classes() and max(g["spread_max"]) > 0
+    assert all(b <= P.SPREAD_MAX for b in g["spread_max"])
+    assert all(c > 0 for b, c in zip(g["spread_max"], g["scale_max"]) if b >= 0)
     assert out["calibration_pass_fraction"] > 0.02
+    assert out["calibration_pass_fraction"] >= out["one_class_gate"]["calibration_pass_fraction"]
     assert out["calibration_max_err_served"] <= 0.5 * BAR
     assert out["heldout_pass_fraction"] > 0.02 and out["heldout_max_err_served"] <= BAR
 
@@ -184,9 +189,9 @@ def test_serving_1m_full_size(ctx, fitted):
     ref, st = d_ref.cpu().numpy(), d_st.cpu().numpy()
     d_p = torch.empty(n, dtype=torch.float64, device="cuda")
     d_r = torch.empty(n, dtype=torch.int32, device="cuda")
+    gate = P.to_gate(out["gate"])
     t0 = time.perf_counter()
-    nfd = sur.price(d_o.data_ptr(), n, out["spread_bound"], fd, d_p.data_ptr(), d_r.data_ptr(),
-                    out["scale_bound"])
+    nfd = sur.price_gated(d_o.data_ptr(), n, gate, fd, d_p.data_ptr(), d_r.data_ptr())
     t_sur = time.perf_counter() - t0
     p, route = d_p.cpu().numpy(), d_r.cpu().numpy()
     served = route == 0
@@ -224,8 +229,8 @@ def test_serving_1m_full_size(ctx, fitted):
         return min(ts)
     t_fd = best(lambda: ctx.fd1d_price_device(d_o.data_ptr(), n, d_ref.data_ptr(),
                                               d_st.data_ptr(), fd))
-    t_sf = best(lambda: sur.price(d_o.data_ptr(), n, out["spread_bound"], fd, d_p.data_ptr(),
-                                  d_r.data_ptr(), out["scale_bound"]))
+    t_sf = best(lambda: sur.price_gated(d_o.data_ptr(), n, gate, fd, d_p.data_ptr(),
+                                        d_r.data_ptr()))
     _say(f"  plain FP64 FD {t_fd * 1e3:.1f} ms, surrogate-first {t_sf * 1e3:.1f} ms: "
          f"{t_fd / t_sf:.3f}x with {served.mean():.3f} of the batch served")
     assert served.mean() >= 0.04

@@ -108,3 +108,46 @@ def test_small_saving_serves_nothing():
                                    min_gain=2 * served) == (-1.0, -1.0)
     assert pipeline.calibrate_gate(err, spread, scale, bar=1e-4, cap=1e-5,
                                    min_gain=served / 2)[0] == thr
+
+
+def test_classes_calibrate_apart():
+    """Two classes, one with errors ten times the other's: the one gate is held to the
+    noisy class's errors, while calibrated apart the quiet class serves far more. Every
+    class keeps every calibration error it serves within half the bar, a class with too
+    few held-out options stays closed, and a total saving under min_gain closes all."""
+    rng = np.random.default_rng(6)
+    n = 400_000
+    cls = rng.integers(0, 2, n)  # class 0 quiet, 1 noisy; classes 2.. are empty
+    spread = rng.uniform(0.0, 5e-5, n)
+    err = spread * rng.normal(0.0, 0.2, n) * np.where(cls == 0, 1.0, 10.0)
+    scale = np.zeros(n)
+    g = pipeline.calibrate_classes(err, spread, scale, cls, t_edges=(0.5,), m_edges=(),
+                                   bar=1e-4, cap=5e-5, min_count=1000)
+    assert len(g["spread_max"]) == 4 and g["spread_max"][2:] == [-1.0, -1.0]
+    b0, b1 = g["spread_max"][:2]
+    assert b0 > b1 > 0
+    for c, b in ((0, b0), (1, b1)):
+        m = (cls == c) & (spread <= b)
+        assert m.sum() >= 1000 and np.abs(err[m]).max() <= 0.5e-4
+    one, _ = pipeline.calibrate_gate(err, spread, scale, bar=1e-4, cap=5e-5)
+    assert one <= b1 < b0
+    served = np.mean(spread <= np.where(cls == 0, b0, b1))
+    assert served > 1.5 * np.mean(spread <= one)
+    closed = pipeline.calibrate_classes(err, spread, scale, cls, t_edges=(0.5,), m_edges=(),
+                                        bar=1e-4, cap=5e-5, min_count=1000, min_gain=2 * served)
+    assert max(closed["spread_max"]) == -1.0
+
+
+def test_gate_mask_follows_the_class_bounds():
+    import kwgpu
+    o = np.zeros(6, dtype=kwgpu.OPTION_DTYPE)
+    o["k"], o["s"] = 100.0, [90.0, 110.0, 90.0, 110.0, 100.0, 100.0]
+    o["t"], o["w"] = [0.2, 0.2, 3.0, 3.0, 1.0, 1.0], [-1, -1, 1, 1, -1, 1]
+    g = {"t_edges": [0.5, 2.0], "m_edges": [0.0], "spread_max": [1e-5] * 12,
+         "scale_max": [0.05] * 12}
+    g["spread_max"][(1 * 3 + 2) * 2 + 1] = -1.0  # calls, T >= 2, S >= K: closed
+    spread = np.full(6, 5e-6)
+    scale = np.full(6, 0.01)
+    assert pipeline.gate_mask(g, o, spread, scale).tolist() == [True, True, True, False, True, True]
+    scale[0] = 0.1
+    assert not pipeline.gate_mask(g, o, spread, scale)[0]

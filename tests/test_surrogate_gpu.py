@@ -333,6 +333,48 @@ def test_serving_routes_and_fallback(ctx, tmp_path):
     s2.close()
 
 
+def test_gated_serving_by_class(ctx):
+    """kwgpu_surrogate_price_gated: with one class it is kwgpu_surrogate_price bit for bit;
+    with classes, an in-box option is served exactly when its class is open (infinite
+    bounds there, closed elsewhere), classes assigned as kwgpu.gate_class assigns them; a
+    malformed gate is refused."""
+    import torch
+    _say("serving by class")
+    o = W.serving(4096, 47)
+    n = len(o)
+    d_o = _device(o)
+    s = kwgpu.Surrogate(ctx, members=2, seed=6)
+    fd = kwgpu.params(256, 256)
+    s.fit(d_o.data_ptr(), n, fd, epochs=1, batch=1024)  # the configuration it emulates
+    bufs = [(torch.empty(n, dtype=torch.float64, device="cuda"),
+             torch.empty(n, dtype=torch.int32, device="cuda")) for _ in range(3)]
+    (p1, r1), (p2, r2), (p3, r3) = bufs
+    nfd1 = s.price(d_o.data_ptr(), n, 1e300, fd, p1.data_ptr(), r1.data_ptr(), 1e300)
+    g1 = kwgpu.gate([], [], [1e300] * 2, [1e300] * 2)
+    nfd2 = s.price_gated(d_o.data_ptr(), n, g1, fd, p2.data_ptr(), r2.data_ptr())
+    a1, a2 = p1.cpu().numpy(), p2.cpu().numpy()
+    assert nfd1 == nfd2 and np.array_equal(a1, a2, equal_nan=True)
+    assert np.array_equal(r1.cpu().numpy(), r2.cpu().numpy())
+    te, me = [0.5, 2.0], [0.0]
+    open_ = np.arange(12) % 2 == 1  # calls or puts, every T band, S >= K only
+    g = kwgpu.gate(te, me, np.where(open_, 1e300, -1.0), np.full(12, 1e300))
+    _, _, ib = _predict(s, d_o, n, fd)
+    want = (ib == 1) & open_[kwgpu.gate_class(o, te, me)]
+    assert 0 < want.sum() < (ib == 1).sum()
+    nfd = s.price_gated(d_o.data_ptr(), n, g, fd, p3.data_ptr(), r3.data_ptr())
+    route = r3.cpu().numpy()
+    assert np.array_equal(route == 0, want) and nfd == n - int(want.sum())
+    assert np.array_equal(p3.cpu().numpy()[want], a1[want])
+    bad = kwgpu.gate([2.0, 0.5], me, [1.0] * 12, [1.0] * 12)  # edges not increasing
+    with pytest.raises(kwgpu.KwgpuError):
+        s.price_gated(d_o.data_ptr(), n, bad, fd, p3.data_ptr(), r3.data_ptr())
+    bad = kwgpu.gate([], [], [1.0] * 2, [1.0] * 2)
+    bad.n_t = kwgpu.GATE_MAX_T + 1
+    with pytest.raises(kwgpu.KwgpuError):
+        s.price_gated(d_o.data_ptr(), n, bad, fd, p3.data_ptr(), r3.data_ptr())
+    s.close()
+
+
 def test_serving_per_option_grids(ctx):
     """The FD fallback solves each routed option on its own grid: the compacted subset
     carries its options' sizes, as the uneven-batch pricer takes them."""

@@ -884,24 +884,34 @@ __global__ __launch_bounds__(kGateBlock) void compact_routes(const int32_t* rout
 
 // Serving candidates: in the training box, with a labelled bracket, in a class the gate
 // leaves open, and with a baseline scale baw + 1e-4 K (the price scale of the network's
-// error) of at most that class's scale bound; flag[i] = 1 for a candidate, else 0, with
-// per-block counts as gate_kernel keeps them (so that compact_routes lists the
-// candidates in index order).
-__global__ __launch_bounds__(kGateBlock) void candidate_kernel(const kwgpu_option* opts, size_t n,
+// error) of at most that class's scale bound; flag[i] = 1 for a candidate, else 0.
+// Launched in blocks of kCandBlock: an option's work (the baseline's critical spot) is
+// long and a batch can be a few ten thousand options, so small blocks spread it over
+// every CU (in 256-thread blocks a 32768-option batch left half the CUs idle).
+constexpr int kCandBlock = 64;
+
+__global__ __launch_bounds__(kCandBlock) void candidate_kernel(const kwgpu_option* opts, size_t n,
                                                                GridArg g, const kwgpu_gate gate,
-                                                               int32_t* flag, uint32_t* blk_counts) {
+                                                               int32_t* flag) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const kwgpu_option o = opts[i];
+    float f[F];
+    const bool in = option_features(o, g.x(i), g.t(i), f);
+    const int k = gate_class(gate, o);
+    int c = 0;
+    if (in && gate.spread_max[k] >= 0.0 && node_pair(o, g.x(i), g.t(i), g.density, g.scale).ok)
+        c = baw_premium(o) + kEps * o.k <= gate.scale_max[k] ? 1 : 0;
+    flag[i] = c;
+}
+
+// Per-block counts of the candidates (blocks of kGateBlock options), as gate_kernel keeps
+// its route counts, so that compact_routes lists the candidates in index order.
+__global__ __launch_bounds__(kGateBlock) void count_candidates(const int32_t* flag, size_t n,
+                                                               uint32_t* blk_counts) {
     __shared__ uint32_t wc[kGateBlock / 64];
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int c = 0;
-    if (i < n) {
-        const kwgpu_option o = opts[i];
-        float f[F];
-        const bool in = option_features(o, g.x(i), g.t(i), f);
-        const int k = gate_class(gate, o);
-        if (in && gate.spread_max[k] >= 0.0 && node_pair(o, g.x(i), g.t(i), g.density, g.scale).ok)
-            c = baw_premium(o) + kEps * o.k <= gate.scale_max[k] ? 1 : 0;
-        flag[i] = c;
-    }
+    const int c = i < n ? flag[i] : 0;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const uint64_t v = __ballot(c == 1);
     if (lane == 0) wc[w] = (uint32_t)__popcll(v);
@@ -1633,7 +1643,9 @@ int kwgpu_surrogate_predict(kwgpu_ctx* ctx, kwgpu_surrogate* s, const kwgpu_opti
                            d_opts + c0, cn, g, s->x, d_in_box ? d_in_box + c0 : nullptr);
         // the whole ensemble at once on the shared features
         HIPCHK(forward(s, 0, s->members, (int)(2 * cn), true, s->yall, (long)(2 * cn), false, st));
-        hipLaunchKernelGGL(node_combine_kernel, dim3(nblk(cn, 256)), dim3(256), 0, st,
+        // one candidate per thread, each solving its baseline's critical spot: blocks of
+        // 64 spread a few thousand candidates over every CU
+        hipLaunchKernelGGL(node_combine_kernel, dim3(nblk(cn, 64)), dim3(64), 0, st,
                            d_opts + c0, cn, g, s->yall, s->members, 2 * cn, d_premium + c0,
                            d_spread + c0);
         HIPCHK(hipGetLastError());
@@ -1762,8 +1774,9 @@ int price_gated(kwgpu_ctx* ctx, kwgpu_surrogate* s, const kwgpu_option* d_opts, 
         // the ensemble runs on the candidates only: listed in index order, gathered with
         // their grid sizes, predicted, and their predictions scattered back
         const GridArg g = grid_arg(s, fd);
-        hipLaunchKernelGGL(candidate_kernel, dim3(nb), dim3(kGateBlock), 0, st, d_opts, n, g,
-                           gate, d_route, blk);
+        hipLaunchKernelGGL(candidate_kernel, dim3(nblk(n, kCandBlock)), dim3(kCandBlock), 0, st,
+                           d_opts, n, g, gate, d_route);
+        hipLaunchKernelGGL(count_candidates, dim3(nb), dim3(kGateBlock), 0, st, d_route, n, blk);
         hipLaunchKernelGGL(scan_block_counts, dim3(1), dim3(1024), 0, st, blk, nb, cnt);
         hipLaunchKernelGGL(compact_routes, dim3(nb), dim3(kGateBlock), 0, st, d_route, n, blk,
                            (const int*)cnt, idx);

@@ -121,10 +121,19 @@ def test_loss_and_gradient_match_oracle(sur, n):
 
 
 def test_gradient_close_to_fp32_autograd(sur):
+    """The kernels' gradient against torch's FP32 autograd of the same network, which has
+    no BF16 rounding: close over all parameters, and slice by slice (each layer's weights
+    and biases), so that a wrong slice -- a bias gradient summed over the wrong axis, a
+    missing factor -- fails on its own."""
     torch = pytest.importorskip("torch")
     rng = np.random.default_rng(5)
     n = 1024
     w = sur.get_params(1)
+    # a non-trivial output layer and biases, so that every slice carries signal
+    w[kW5:kB5] = rng.uniform(-0.3, 0.3, H).astype(np.float32)
+    for b in (kB1, kB2, kB3, kB4):
+        w[b:b + H] = rng.uniform(-0.1, 0.1, H).astype(np.float32)
+    sur.set_params(1, w)
     x = rng.uniform(-1, 1, (n, F)).astype(np.float32)
     t = rng.normal(0, 0.5, n).astype(np.float32)
     _, g = sur.loss_grad(1, x, t)
@@ -139,8 +148,29 @@ def test_gradient_close_to_fp32_autograd(sur):
     loss = torch.mean((y - torch.tensor(t)) ** 2)
     loss.backward()
     ref = tw.grad.numpy().astype(np.float64)
+    g = g.astype(np.float64)
     cos = np.dot(ref, g) / (np.linalg.norm(ref) * np.linalg.norm(g))
     assert cos > 0.995, cos
+
+    def slice_ok(gg, name):
+        a, b = SLICES[name]
+        ga, gr = gg[a:b], ref[a:b]
+        c = np.dot(ga, gr) / (np.linalg.norm(ga) * np.linalg.norm(gr))
+        ratio = np.linalg.norm(ga) / np.linalg.norm(gr)
+        return c, ratio, c > 0.999 and 0.98 < ratio < 1.02
+
+    for name in SLICES:
+        c, ratio, ok = slice_ok(g, name)
+        _say(f"  {name}: cos {c:.5f}, norm ratio {ratio:.4f}")
+        assert ok, name
+    # the check rejects the mistakes it is there for: a bias gradient twice too large,
+    # and another layer's bias gradient in its place
+    bad = g.copy()
+    bad[kB2:kW3] *= 2.0
+    assert not slice_ok(bad, "b2")[2]
+    bad = g.copy()
+    bad[kB3:kW4] = g[kB4:kW5]
+    assert not slice_ok(bad, "b3")[2]
 
 
 def test_baseline_matches_numpy_restatement(ctx):

@@ -123,7 +123,10 @@ def calibrate_spread(err, spread, bar=ERR_BAR, cap=SPREAD_MAX, margin=0.5,
 SCALE_CAPS = (0.01, 0.015, 0.02, 0.03, 0.05, 0.1, float("inf"))
 
 
-MIN_GAIN = 0.01  # serve only when the gate saves at least this share of the FD solves
+# Serve only when the gate saves at least this share of the FD solves: the serving path's
+# fixed costs (two count read-backs, the compactions) eat smaller savings -- a per-option
+# gate expected to save 1.2 % measured 0.998x plain FD.
+MIN_GAIN = 0.02
 
 
 def calibrate_gate(err, spread, scale, bar=ERR_BAR, cap=SPREAD_MAX, caps=SCALE_CAPS, cost=0.0,

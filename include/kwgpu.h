@@ -233,9 +233,10 @@ void* kwgpu_stream(kwgpu_ctx* ctx);
  * one x_dim x t_dim grid or per-option grids. An ensemble of MLPs 8 -> 256 -> 256 ->
  * 256 -> 256 -> 1 with SiLU predicts, at a mesh node x_j of an option's own grid, the
  * FD value there minus the Black-Scholes price, as y = (premium - baseline) /
- * (baseline + 1e-4 K), where baseline is the Barone-Adesi & Whaley quadratic
- * approximation of the premium (kwgpu_surrogate_baseline; a control variate whose size
- * sets the price scale of the network's error). Features: log(S/K), T, sigma, r, q, w
+ * (baseline + 1e-4 K), where baseline is Ju & Zhong's (1999) correction of the
+ * Barone-Adesi & Whaley quadratic approximation of the premium
+ * (kwgpu_surrogate_baseline; a control variate whose size sets the price scale of the
+ * network's error). Features: log(S/K), T, sigma, r, q, w
  * normalised to the box K/S in [0.4, 2], T in [1/52, 5], sigma in [0.1, 0.6],
  * r in [0, 0.1], q in [0, 0.08], and log2 of the grid's x and t points.
  * A price is formed as the FD pricer forms its own (Fd1d::value,
@@ -378,8 +379,8 @@ int kwgpu_surrogate_loss_grad(kwgpu_ctx* ctx, kwgpu_surrogate* s, int32_t member
                               const float* host_features, const float* host_targets, size_t n,
                               float* host_loss, float* host_grad);
 
-/* The analytic baseline premium (Barone-Adesi & Whaley, continuous dividend yield) of
- * each option, host memory; price units. */
+/* The analytic baseline premium (Ju & Zhong's correction of Barone-Adesi & Whaley,
+ * continuous dividend yield) of each option, host memory; price units. */
 int kwgpu_surrogate_baseline(kwgpu_ctx* ctx, const kwgpu_option* host_opts, size_t n,
                              double* host_out);
 

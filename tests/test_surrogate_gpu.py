@@ -174,14 +174,15 @@ def test_gradient_close_to_fp32_autograd(sur):
 
 
 def test_baseline_matches_numpy_restatement(ctx):
-    """The device's Barone-Adesi & Whaley premium against tests/baw_ref.py, including
-    the no-early-exercise cases (puts at r = 0, calls at q = 0)."""
-    from baw_ref import baw_premium
+    """The device's baseline premium (Ju & Zhong's correction of Barone-Adesi & Whaley)
+    against tests/baw_ref.py, including the no-early-exercise cases (puts at r = 0,
+    calls at q = 0)."""
+    from baw_ref import baseline_premium
     o = W.surrogate_train(4096, 12)
     o["r"][:64] = 0.0     # puts: no premium; calls with q > 0: the r -> 0 limit
     o["q"][64:128] = 0.0  # calls: no premium
     got = kwgpu.surrogate_baseline(ctx, o)
-    ref = baw_premium(o)
+    ref = baseline_premium(o)
     assert np.all(np.isfinite(got))
     assert np.max(np.abs(got - ref) / o["k"]) < 1e-9
     assert np.all(got[:64][o["w"][:64] < 0] == 0) and np.all(got[64:128][o["w"][64:128] > 0] == 0)
@@ -262,10 +263,10 @@ def test_training_learns_the_premium(ctx):
 def test_node_form_matches_fd_interpolation(ctx):
     """The surrogate forms a price from its two node values as the FD pricer does
     (src/Math/kwFd1d.cpp:139-158). With the network's weights zeroed except the output
-    bias, the premium at every node is baw + (baw + 1e-4 K) b5, so the price is BS plus
-    that, interpolated with the reference's weights at the bracketing nodes (computed
-    here from the sinh mesh) -- to rounding."""
-    from baw_ref import baw_premium
+    bias, the premium at every node is base + (base + 1e-4 K) b5 (base: the analytic
+    baseline), so the price is BS plus that, interpolated with the reference's weights
+    at the bracketing nodes (computed here from the sinh mesh) -- to rounding."""
+    from baw_ref import baseline_premium
     from node_ref import DAMP_MIN, bracket, kink_damping, nodes
     o = W.surrogate_train(2000, 77)
     fd = kwgpu.params(512, 256)
@@ -295,8 +296,8 @@ def test_node_form_matches_fd_interpolation(ctx):
     ea, eb, es = a.copy(), b.copy(), o.copy()
     for q in (ea, eb, es):
         q["e"] = 0
-    va = ctx.bs_price(ea) + baw_premium(a) + (baw_premium(a) + 1e-4 * o["k"]) * b5
-    vb = ctx.bs_price(eb) + baw_premium(b) + (baw_premium(b) + 1e-4 * o["k"]) * b5
+    va = ctx.bs_price(ea) + baseline_premium(a) + (baseline_premium(a) + 1e-4 * o["k"]) * b5
+    vb = ctx.bs_price(eb) + baseline_premium(b) + (baseline_premium(b) + 1e-4 * o["k"]) * b5
     want = ((x1 - xs) * va + (xs - x0) * vb) / (x1 - x0) - ctx.bs_price(es)
     d = np.abs(pm[ok] - want[ok])
     _say("  node form vs restatement: max", d.max())

@@ -307,6 +307,76 @@ def test_node_form_matches_fd_interpolation(ctx):
     s.close()
 
 
+def oracle_forward(w, x):
+    """The network's output y for features x: oracle_loss_grad's forward pass."""
+    p = unpack(w)
+    h = bf16(silu(x.astype(np.float64) @ p["W1"].T + p["b1"]))
+    for l in (2, 3):
+        h = bf16(silu(h @ bf16(p[f"W{l}"]).T + p[f"b{l}"]))
+    return silu(h @ bf16(p["W4"]).T + p["b4"]) @ p["W5"] + p["b5"]
+
+
+def test_prediction_matches_forward_oracle(ctx):
+    """Prediction runs the whole network in one kernel per 128-row tile and member. Its
+    premium and spread against the float64 forward restatement (BF16 rounding where
+    the kernels round) at each option's two bracketing nodes, combined as the FD pricer
+    interpolates: two members with different random weights, and 2 x 1500 node rows, so
+    the last tile is partial. A wrong member or layer offset, a transposed weight or a
+    dropped bias moves the network's share of the premium by far more than 3e-3 of
+    itself. FP32 accumulation against float64 decides some BF16 rounding ties the
+    other way, and those ties compound over three layers: measured up to 8e-4."""
+    from baw_ref import baseline_premium
+    from node_ref import bracket, nodes
+    o = W.surrogate_train(1500, 79)
+    fd = kwgpu.params(512, 256)
+    s = kwgpu.Surrogate(ctx, members=2, seed=5)
+    d_o = _device(o)
+    s.fit(d_o.data_ptr(), len(o), fd, epochs=1, batch=1024)  # records the configuration
+    rng = np.random.default_rng(21)
+    ws = []
+    for m in range(2):
+        w = s.get_params(m)
+        w[kW5:kB5] = rng.uniform(-0.3, 0.3, H).astype(np.float32)
+        w[kB5] = rng.uniform(-0.5, 0.5)
+        for b0, b1 in ((kB1, kW2), (kB2, kW3), (kB3, kW4), (kB4, kW5)):
+            w[b0:b1] = rng.uniform(-0.1, 0.1, b1 - b0).astype(np.float32)
+        s.set_params(m, w)
+        ws.append(w)
+    pm, sp, ib = _predict(s, d_o, len(o), fd)
+    ok = ib == 1
+    assert ok.mean() > 0.3
+    xs = np.log(o["s"] / o["k"])
+    g = nodes(o, 512)
+    jj = np.minimum(bracket(g, xs), 511)
+    r = np.arange(len(o))
+    x0, x1 = g[r, jj - 1], g[r, jj]
+    a, b = o.copy(), o.copy()
+    a["s"], b["s"] = o["k"] * np.exp(x0), o["k"] * np.exp(x1)
+    ea, eb, es = a.copy(), b.copy(), o.copy()
+    for q in (ea, eb, es):
+        q["e"] = 0
+    ba, bb = baseline_premium(a), baseline_premium(b)
+    sa, sb = ba + 1e-4 * o["k"], bb + 1e-4 * o["k"]
+    fa, fb = kwgpu.surrogate_features(a, 512, 256), kwgpu.surrogate_features(b, 512, 256)
+    v, net = [], np.zeros(len(o))
+    for w in ws:
+        ya, yb = oracle_forward(w, fa), oracle_forward(w, fb)
+        va = ctx.bs_price(ea) + ba + sa * ya
+        vb = ctx.bs_price(eb) + bb + sb * yb
+        v.append(((x1 - xs) * va + (xs - x0) * vb) / (x1 - x0))
+        net += np.abs(sa * ya) + np.abs(sb * yb)
+    want_pm = 0.5 * (v[0] + v[1]) - ctx.bs_price(es)
+    want_sp = np.abs(v[0] - v[1])
+    tol = 3e-3 * net + 1e-9 * o["k"]
+    d_pm, d_sp = np.abs(pm - want_pm)[ok], np.abs(sp - want_sp)[ok]
+    _say(f"  fused prediction vs oracle: premium {np.max(d_pm / tol[ok]):.2e} of the "
+         f"tolerance, spread {np.max(d_sp / tol[ok]):.2e}; spread rms {np.sqrt(np.mean(want_sp[ok] ** 2)):.2e}")
+    assert np.all(d_pm <= tol[ok]) and np.all(d_sp <= tol[ok])
+    # the members differ, so the spread is not the trivial 0
+    assert np.median(want_sp[ok]) > 30 * np.median(tol[ok])
+    s.close()
+
+
 def test_serving_routes_and_fallback(ctx, tmp_path):
     import torch
     _say("serving")

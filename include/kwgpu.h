@@ -232,13 +232,13 @@ void* kwgpu_stream(kwgpu_ctx* ctx);
  * A surrogate emulates one FD pricer configuration: density, scale, theta and either
  * one x_dim x t_dim grid or per-option grids. An ensemble of MLPs 8 -> 256 -> 256 ->
  * 256 -> 256 -> 1 with SiLU predicts, at a mesh node x_j of an option's own grid, the
- * FD value there minus the Black-Scholes price, as y = (premium - baseline) /
- * (baseline + 1e-4 K), where baseline is Ju & Zhong's (1999) correction of the
- * Barone-Adesi & Whaley quadratic approximation of the premium
- * (kwgpu_surrogate_baseline; a control variate whose size sets the price scale of the
- * network's error). Features: log(S/K), T, sigma, r, q, w
- * normalised to the box K/S in [0.4, 2], T in [1/52, 5], sigma in [0.1, 0.6],
- * r in [0, 0.1], q in [0, 0.08], and log2 of the grid's x and t points.
+ * FD value there minus the Black-Scholes price, as y = (premium - base) /
+ * (base + 1e-4 K), where base is the premium from the option's exercise boundary,
+ * solved on the device by the boundary's integral-equation fixed point (a control
+ * variate, whose size sets the price scale of the network's error). Features: log(S/K),
+ * T, sigma, r, q, w normalised to the box K/S in [0.4, 2], T in [1/52, 5],
+ * sigma in [0.1, 0.6], r in [0, 0.1], q in [0, 0.08], and log2 of the grid's x and t
+ * points.
  * A price is formed as the FD pricer forms its own (Fd1d::value,
  * src/Math/kwFd1d.cpp:139-158): the network at the two nodes bracketing log(S/K), BS
  * added at each, linear interpolation with the reference's weights. The labels are the
@@ -316,8 +316,8 @@ int kwgpu_surrogate_predict(kwgpu_ctx* ctx, kwgpu_surrogate* s, const kwgpu_opti
 /* Surrogate-first pricing: options in the box whose ensemble spread is <= spread_max
  * are priced by the surrogate (BS + premium); the rest go to the FD pricer with `fd`.
  * The ensemble runs only on candidates: in-box options with a labelled bracket whose
- * baseline scale (baseline premium + 1e-4 K, the price scale of the network's error) is
- * at most scale_max (INFINITY: every in-box option); the others go to FD directly
+ * baseline scale (kwgpu_surrogate_baseline's premium + 1e-4 K, the price scale of the
+ * network's error) is at most scale_max (INFINITY: every in-box option); the others go to FD directly
  * (fd->precision FP64, or FP32_MIXED with the options beyond the mixed march's limits,
  * see KWGPU_FP32_MIXED, sent to FP64). fd->x_dims / fd->t_dims, when set, are device arrays of
  * n per-option grid sizes, and each routed option is solved on its own. The surrogate
@@ -379,8 +379,9 @@ int kwgpu_surrogate_loss_grad(kwgpu_ctx* ctx, kwgpu_surrogate* s, int32_t member
                               const float* host_features, const float* host_targets, size_t n,
                               float* host_loss, float* host_grad);
 
-/* The analytic baseline premium (Ju & Zhong's correction of Barone-Adesi & Whaley,
- * continuous dividend yield) of each option, host memory; price units. */
+/* The analytic baseline premium of each option, host memory, price units: Ju & Zhong's
+ * correction of Barone-Adesi & Whaley, continuous dividend yield. The gate's scale is
+ * this premium + 1e-4 K. */
 int kwgpu_surrogate_baseline(kwgpu_ctx* ctx, const kwgpu_option* host_opts, size_t n,
                              double* host_out);
 

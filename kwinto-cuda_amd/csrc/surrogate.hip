@@ -4,8 +4,8 @@
 // one x/t grid or per-option grids). An ensemble of MLPs 8 -> 256 -> 256 -> 256 -> 256
 // -> 1 (SiLU) predicts, at a mesh node x_j of an option's own FD grid, the FD solution
 // there minus the Black-Scholes price, as y = (premium - base) / (base + 1e-4 K) against
-// an analytic approximation base of that premium (Ju & Zhong's correction of Barone-Adesi
-// & Whaley). A price is then formed exactly as the FD pricer forms its own
+// a control variate base: the premium from the option's exercise boundary, solved by a
+// fixed point (boundary_setup). A price is then formed exactly as the FD pricer forms its own
 // (Fd1d::value, src/Math/kwFd1d.cpp:139-158): the network is
 // evaluated at the two nodes that bracket log(S/K), BS is added at each node, and the
 // two node values are interpolated linearly with the reference's weights. The
@@ -52,18 +52,20 @@ constexpr int kPartRows = 128, kPartWidth = H * F + H;
 constexpr int kMaxSplits = 32;
 constexpr size_t kPartFloats = std::max<size_t>((size_t)kPartRows * kPartWidth,
                                                 (size_t)kMaxSplits * H * H);
-// Target: y = (premium - base) / (base + kEps K), the part of the early-exercise premium
-// that the analytic baseline below (base) misses, relative to that baseline (floored at
-// kEps K). A network error dy then costs (base + kEps K) dy in price: small
-// where the premium is small, which is where an absolute error bar can be met.
+// Target: y = (premium - base) / (base + kEps K), the part of the FD pricer's premium
+// that the control variate (base, the premium from the exercise boundary below) misses,
+// relative to it (floored at kEps K). A network error dy then costs (base + kEps K) dy in
+// price: small where the premium is small, which is where an absolute error bar can be
+// met.
 constexpr double kEps = 1e-4;
-constexpr uint32_t kFileVersion = 5;  // saved parameters belong to this target and features
+constexpr uint32_t kFileVersion = 6;  // saved parameters belong to this target and features
 
-// The baseline: Ju & Zhong's (1999) correction of the Barone-Adesi & Whaley (1987)
-// quadratic approximation of the early-exercise premium (American minus European
-// Black-Scholes), with a continuous dividend yield q. With alpha = 2r / sigma^2,
-// beta = 2 (r - q) / sigma^2, h = 1 - e^{-rT}, m = alpha / h (-> 2 / (sigma^2 T) as
-// r -> 0) and D = (beta - 1)^2 + 4m,
+// The gate's price scale: Ju & Zhong's (1999) correction of the Barone-Adesi & Whaley
+// (1987) quadratic approximation of the early-exercise premium (American minus European
+// Black-Scholes), with a continuous dividend yield q. It is cheap (one critical spot per
+// option), so the candidate gate takes it for every option of a batch. With
+// alpha = 2r / sigma^2, beta = 2 (r - q) / sigma^2, h = 1 - e^{-rT}, m = alpha / h
+// (-> 2 / (sigma^2 T) as r -> 0) and D = (beta - 1)^2 + 4m,
 //     lambda = (-(beta - 1) + w sqrt(D)) / 2,
 // the critical spot S* solves (Barone-Adesi & Whaley)
 //     w (S* - K) = E(S*) + w (1 - e^{-qT} Phi(w d1(S*))) S* / lambda,
@@ -81,9 +83,9 @@ constexpr uint32_t kFileVersion = 5;  // saved parameters belong to this target 
 // the factor 1 / (1 - chi) is replaced by 1 + chi, equal to first order: it stays
 // positive where chi reaches 1 (options with r or q near 0, whose premium is near 0).
 // Beyond S* the premium is w (S - K) - E(S). It is 0 where early exercise never pays
-// (calls with q <= 0, puts with r <= 0). A control variate only: the network learns the
-// remainder. Against the FD price at 1024^2 on 3000 options of the training box, the
-// correction leaves an rms premium error of 0.056 against BAW's 0.32.
+// (calls with q <= 0, puts with r <= 0). Against the FD price at 1024^2 on 3000 options
+// of the training box, the correction leaves an rms premium error of 0.056 against
+// BAW's 0.32. Its critical spot also starts the exercise-boundary fixed point below.
 // S* and the coefficients do not depend on the spot, so they are set up once per option
 // (baseline_setup) and the premium taken at any spot from them (baseline_at): the
 // surrogate evaluates it at the option's spot and at the two mesh nodes around it.
@@ -191,6 +193,304 @@ __host__ __device__ inline double baseline_at(const kwgpu_option& o, const Basel
 
 __host__ __device__ inline double baseline_premium(const kwgpu_option& o) {
     return baseline_at(o, baseline_setup(o), o.s);
+}
+
+// ---- control variate: the premium from the exercise boundary ------------------------
+// The early-exercise premium of the American put with unit strike, rate r, yield q and
+// volatility sigma, from its exercise boundary b(tau) (tau: time to maturity). The
+// boundary solves, for every tau (Andersen, Lake & Offengenden 2016, their fixed point
+// "A"; it is the premium representation below taken at S = b(tau)),
+//     b(tau) = e^{-(r-q) tau} N(tau) / D(tau),
+//     N = Phi(d-(tau, b(tau))) + r int_0^tau e^{ru} Phi(d-(tau - u, b(tau) / b(u))) du,
+//     D = Phi(d+(tau, b(tau))) + q int_0^tau e^{qu} Phi(d+(tau - u, b(tau) / b(u))) du,
+// d+-(t, z) = (ln z + (r - q) t +- sigma^2 t / 2) / (sigma sqrt t), and the premium at a
+// spot S above b(T) is
+//     int_0^T [r e^{-r(T-u)} Phi(-d-(T-u, S / b(u))) - q S e^{-q(T-u)} Phi(-d+(T-u, S / b(u)))] du
+// (at or below b(T): 1 - S minus the European put). b(0+) = X = min(1, r / q) (1 when
+// q <= 0). b is held as H(xi) = ln(b / X)^2 on kBn + 1 Chebyshev-Lobatto nodes in
+// xi = sqrt(tau) over [0, sqrt T], where H is smooth. The fixed point takes kBit Jacobi
+// steps from b_i = X (b_T / X)^{sqrt(tau_i / T)}, b_T the Barone-Adesi & Whaley critical
+// spot at T; the integrals take a 17-point tanh-sinh rule on u = tau a_k, which holds
+// the sqrt and log behaviour of the integrands at both ends. Calls price as puts by
+// put-call symmetry: C(S, K; r, q) = P(K, S; q, r), and P(S, K) = K p(S / K) for the
+// unit-strike premium p. Against the FD price at 1024^2 on 3000 options of the training
+// box, the remainder FD - premium has rms 1.7e-3 -- the FD solver's own discretisation
+// error, since the same at 16 nodes and 16 steps changes it by under 1e-4 -- against
+// 0.064 for Ju & Zhong. No early exercise (0): puts with r <= 0, calls with q <= 0.
+constexpr int kBn = 6, kBit = 8, kTs = 17;
+// tanh-sinh abscissae a_k = (1 + tanh(pi/2 sinh(k h))) / 2 and weights, h = 0.375, |k| <= 8
+// (a_{16-k} = 1 - a_k to the last digit, so tau - u is taken from the mirror point)
+__host__ __device__ inline double ts_a(int k) {
+    constexpr double a[kTs] = {
+        2.1470805279391156e-14, 4.2864745391083903e-10, 3.9763144264366805e-07,
+        4.5307652428000167e-05, 0.0012425717713878076, 0.013016565902161277,
+        0.070215470655051695,   0.23042664730601614,   0.5,
+        0.76957335269398386,    0.92978452934494826,   0.98698343409783873,
+        0.99875742822861224,    0.99995469234757206,   0.99999960236855734,
+        0.99999999957135244,    0.99999999999997846};
+    return a[k];
+}
+__host__ __device__ inline double ts_w(int k) {
+    constexpr double w[kTs] = {
+        2.5465845514760737e-13, 3.5038654253880788e-09, 2.2469440055945116e-06,
+        0.00017811502580315067, 0.0034393438106114739,  0.025766645199968952,
+        0.09957717664276014,    0.22377431734671693,    0.2945243112740431,
+        0.22377431734671693,    0.09957717664276014,    0.025766645199968952,
+        0.0034393438106114739,  0.00017811502580315067, 2.2469440055945116e-06,
+        3.5038654253880788e-09, 2.5465845514760737e-13};
+    return w[k];
+}
+// sqrt(a_k): sqrt(u) = sqrt(tau) sqrt(a_k) and sqrt(tau - u) = sqrt(tau) sqrt(a_{16-k})
+__host__ __device__ inline double ts_sa(int k) {
+    constexpr double s[kTs] = {
+        1.4652919599653563e-07, 2.07038028852392e-05,  0.00063058024282692843,
+        0.0067310959306787601,  0.035250131508801602,  0.11409016566804203,
+        0.26498201949387379,    0.48002775680789139,   0.70710678118654757,
+        0.87725330018985048,    0.96425335329722772,   0.99347039920565261,
+        0.99937852099623004,    0.99997734591718235,   0.99999980118425891,
+        0.99999999978567622,    0.99999999999998923};
+    return s[k];
+}
+// cos(m pi / 6)
+__host__ __device__ inline double cos6(int m) {
+    constexpr double c[12] = {1.0, 0.86602540378443865, 0.5, 0.0, -0.5, -0.86602540378443865,
+                              -1.0, -0.86602540378443865, -0.5, 0.0, 0.5, 0.86602540378443865};
+    return c[m % 12];
+}
+static_assert(kBn == 6, "cos6 holds the Chebyshev-Lobatto angles of six intervals");
+
+__host__ __device__ inline double phi_cdf(double x) {
+    return 0.5 * erfc(-x * 0.70710678118654752440);
+}
+
+struct Boundary {
+    double a[kBn + 1];  // Chebyshev coefficients of H over z = 1 - 2 xi / sqrt T
+    double x0, r, q, z, t, st;  // X, the put's rate and yield, sigma, T, sqrt T
+    bool call, none;
+};
+
+// ln(b(u) / X) = -sqrt(H) at zz = 1 - 2 sqrt(u) / sqrt(T) (H by Clenshaw)
+__host__ __device__ inline double boundary_lnb(const Boundary& bd, double zz) {
+    double b1 = 0.0, b2 = 0.0;
+    for (int k = kBn; k >= 1; --k) {
+        const double b0 = 2.0 * zz * b1 - b2 + bd.a[k];
+        b2 = b1;
+        b1 = b0;
+    }
+    const double h = zz * b1 - b2 + bd.a[0];
+    return -sqrt(fmax(h, 0.0));
+}
+__host__ __device__ inline double boundary_bT(const Boundary& bd) {  // b(T)
+    return bd.x0 * exp(boundary_lnb(bd, -1.0));
+}
+
+// Chebyshev-Lobatto coefficients of H = ln(b / X)^2 from ln(b_i / X) at the nodes
+__host__ __device__ inline void boundary_fit_log(Boundary& bd, const double (&lb)[kBn + 1]) {
+    double hv[kBn + 1];
+    for (int i = 0; i <= kBn; ++i) hv[i] = lb[i] * lb[i];
+    for (int k = 0; k <= kBn; ++k) {
+        double s = 0.5 * (hv[0] + hv[kBn] * cos6(k * kBn));
+        for (int i = 1; i < kBn; ++i) s += hv[i] * cos6(k * i);
+        bd.a[k] = (2.0 / kBn) * s * (k == 0 || k == kBn ? 0.5 : 1.0);
+    }
+}
+__host__ __device__ inline void boundary_fit(Boundary& bd, const double (&b)[kBn + 1]) {
+    double lb[kBn + 1];
+    for (int i = 0; i <= kBn; ++i) lb[i] = log(b[i] / bd.x0);
+    boundary_fit_log(bd, lb);
+}
+
+// Parameters, node times tau_i and the starting boundary b_i.
+__host__ __device__ inline void boundary_start(const kwgpu_option& o, Boundary& bd,
+                                               double (&b)[kBn + 1], double (&tau)[kBn + 1]) {
+    bd.call = !(o.w < 0);
+    bd.r = bd.call ? o.q : o.r;
+    bd.q = bd.call ? o.r : o.q;
+    bd.z = o.z;
+    bd.t = o.t;
+    bd.none = !(o.k > 0 && o.t > 0 && o.z > 0) || !(bd.r > 0);
+    if (bd.none) return;
+    bd.st = sqrt(o.t);
+    bd.x0 = bd.q > 0 ? fmin(1.0, bd.r / bd.q) : 1.0;
+    kwgpu_option u{};  // the unit-strike put, for b_T
+    u.t = o.t;
+    u.k = 1.0;
+    u.z = o.z;
+    u.r = bd.r;
+    u.q = bd.q;
+    u.s = 1.0;
+    u.e = 1;
+    u.w = -1;
+    const double bT = fmin(baseline_setup(u).ss, bd.x0);
+    for (int i = 0; i <= kBn; ++i) {
+        const double f = 0.5 * (1.0 - cos6(i));  // xi_i / sqrt T
+        tau[i] = o.t * f * f;
+        b[i] = bd.x0 * pow(bT / bd.x0, f);
+    }
+    b[0] = bd.x0;
+}
+
+// One fixed-point step at node tau_i (b_i its current value), with bd fitted to the
+// current boundary: each quadrature point k adds its terms to the integrals' sums
+// (boundary_point; er, eq = e^{r u}, e^{q u} at the point), then boundary_next forms the
+// new b_i. In the log domain, ln(b_i / b(u)) = ln(b_i / X) + sqrt(H(u)), so no
+// exponential or logarithm of b(u) is taken.
+struct NodeStep {
+    double ti, xi, f, lbi, c;  // tau_i, sqrt(tau_i), sqrt(tau_i / T), ln(b_i / X), r - q - s^2 / 2
+};
+__host__ __device__ inline NodeStep node_step(const Boundary& bd, double ti, double lbi) {
+    NodeStep n;
+    n.ti = ti;
+    n.xi = sqrt(ti);
+    n.f = n.xi / bd.st;
+    n.lbi = lbi;
+    n.c = bd.r - bd.q - 0.5 * bd.z * bd.z;
+    return n;
+}
+__host__ __device__ inline void boundary_point(const Boundary& bd, const NodeStep& n, int k,
+                                               double er, double eq, double& ni, double& di) {
+    const double dt = n.ti * ts_a(kTs - 1 - k);                // tau - u
+    const double sd = bd.z * n.xi * ts_sa(kTs - 1 - k);        // sigma sqrt(tau - u)
+    const double lb = boundary_lnb(bd, 1.0 - 2.0 * n.f * ts_sa(k));
+    const double dm = (n.lbi - lb + n.c * dt) / sd;
+    ni += ts_w(k) * er * phi_cdf(dm);
+    di += ts_w(k) * eq * phi_cdf(dm + sd);
+}
+__host__ __device__ inline void boundary_sums(const Boundary& bd, double ti, double bi, int k0,
+                                              int kstep, double& ni, double& di) {
+    const NodeStep n = node_step(bd, ti, log(bi / bd.x0));
+    ni = 0.0;
+    di = 0.0;
+    for (int k = k0; k < kTs; k += kstep) {
+        const double u1 = ti * ts_a(k);
+        boundary_point(bd, n, k, exp(bd.r * u1), exp(bd.q * u1), ni, di);
+    }
+}
+__host__ __device__ inline double boundary_next(const Boundary& bd, double ti, double bi,
+                                                double ni, double di) {
+    const double s = bd.z, r = bd.r, q = bd.q;
+    const double sd0 = s * sqrt(ti);
+    const double d0 = (log(bi) + (r - q - 0.5 * s * s) * ti) / sd0;
+    const double v = exp(-(r - q) * ti) * (phi_cdf(d0) + r * ti * ni) /
+                     (phi_cdf(d0 + sd0) + q * ti * di);
+    return fmin(v, bd.x0);
+}
+__host__ __device__ inline double boundary_step(const Boundary& bd, double ti, double bi) {
+    double ni, di;
+    boundary_sums(bd, ti, bi, 0, 1, ni, di);
+    return boundary_next(bd, ti, bi, ni, di);
+}
+
+__host__ __device__ inline Boundary boundary_setup(const kwgpu_option& o) {
+    Boundary bd{};
+    double b[kBn + 1], tau[kBn + 1];
+    boundary_start(o, bd, b, tau);
+    if (bd.none) return bd;
+    for (int it = 0; it < kBit; ++it) {
+        boundary_fit(bd, b);
+        double nb[kBn + 1];
+        for (int i = 1; i <= kBn; ++i) nb[i] = boundary_step(bd, tau[i], b[i]);
+        for (int i = 1; i <= kBn; ++i) b[i] = nb[i];
+    }
+    boundary_fit(bd, b);
+    return bd;
+}
+
+// boundary_setup by a group of kGroup lanes (lane j of the group; groups aligned to
+// kGroup lanes, every lane of a group taking part): node i in 1..kBn is stepped by lanes
+// 2i - 2 and 2i - 1, each summing every other quadrature point, and the new values are
+// exchanged by shuffle before each fit. The same arithmetic as boundary_setup up to the
+// order of the two half sums.
+constexpr int kGroup = 16;
+__device__ inline Boundary boundary_setup_group(const kwgpu_option& o, int j) {
+    static_assert(2 * kBn <= kGroup, "two lanes per node");
+    Boundary bd{};
+    double b[kBn + 1], tau[kBn + 1];
+    boundary_start(o, bd, b, tau);
+    if (bd.none) return bd;  // the same for every lane of the group
+    const int node = j < 2 * kBn ? 1 + j / 2 : 0;
+    double ti = tau[0], bi = b[0];
+#pragma unroll
+    for (int i = 1; i <= kBn; ++i)
+        if (node == i) {
+            ti = tau[i];
+            bi = b[i];
+        }
+    // this lane's quadrature points k = k0 + 2m and their e^{r u}, e^{q u}, which do
+    // not change between steps
+    constexpr int KP = (kTs + 1) / 2;
+    const int k0 = j & 1;
+    double er[KP], eq[KP];
+#pragma unroll
+    for (int m = 0; m < KP; ++m) {
+        const double u1 = ti * ts_a(min(k0 + 2 * m, kTs - 1));
+        er[m] = exp(bd.r * u1);
+        eq[m] = exp(bd.q * u1);
+    }
+    // ln(b_i / X), taken by the lanes of node i and exchanged (ln(b_0 / X) = 0)
+    double lb[kBn + 1];
+    for (int i = 0; i <= kBn; ++i) lb[i] = log(b[i] / bd.x0);
+    double lbi = log(bi / bd.x0);
+    for (int it = 0; it < kBit; ++it) {
+        boundary_fit_log(bd, lb);
+        double ni = 0.0, di = 0.0;
+        if (node) {
+            const NodeStep n = node_step(bd, ti, lbi);
+#pragma unroll
+            for (int m = 0; m < KP; ++m)
+                if (k0 + 2 * m < kTs) boundary_point(bd, n, k0 + 2 * m, er[m], eq[m], ni, di);
+        }
+        ni += __shfl_xor(ni, 1, kGroup);
+        di += __shfl_xor(di, 1, kGroup);
+        if (node) {
+            bi = boundary_next(bd, ti, bi, ni, di);
+            lbi = log(bi / bd.x0);
+        }
+#pragma unroll
+        for (int i = 1; i <= kBn; ++i) lb[i] = __shfl(lbi, 2 * i - 2, kGroup);
+    }
+    boundary_fit_log(bd, lb);
+    return bd;
+}
+
+// The premium at spot s of the option whose boundary is bd (o's own spot is not used):
+// the put's strike kk and unit spot x; at or below b(T) the exercise value minus the
+// European put (boundary_exercised), above it kk T times the integral's sum over the
+// quadrature points (boundary_integral: the points k0, k0 + kstep, ...).
+__host__ __device__ inline void boundary_spot(const kwgpu_option& o, const Boundary& bd, double s,
+                                              double& kk, double& x) {
+    kk = bd.call ? s : o.k;        // the put's strike
+    x = (bd.call ? o.k : s) / kk;  // and its spot, per unit strike
+}
+__host__ __device__ inline double boundary_exercised(const Boundary& bd, double kk, double x) {
+    const double r = bd.r, q = bd.q, sg = bd.z, T = bd.t;
+    const double sd = sg * bd.st;
+    const double dm = (log(x) + (r - q - 0.5 * sg * sg) * T) / sd;
+    const double eu = exp(-r * T) * phi_cdf(-dm) - x * exp(-q * T) * phi_cdf(-dm - sd);
+    return kk * ((1.0 - x) - eu);
+}
+__host__ __device__ inline double boundary_integral(const Boundary& bd, double x, int k0, int kstep) {
+    const double r = bd.r, q = bd.q, sg = bd.z, T = bd.t;
+    const double lx = log(x / bd.x0), c = r - q - 0.5 * sg * sg;
+    double p = 0.0;
+    for (int k = k0; k < kTs; k += kstep) {
+        const double dt = T * ts_a(kTs - 1 - k);            // T - u
+        const double sd = sg * bd.st * ts_sa(kTs - 1 - k);  // sigma sqrt(T - u)
+        const double dm = (lx - boundary_lnb(bd, 1.0 - 2.0 * ts_sa(k)) + c * dt) / sd;
+        p += ts_w(k) * (r * exp(-r * dt) * phi_cdf(-dm) - q * x * exp(-q * dt) * phi_cdf(-dm - sd));
+    }
+    return p;
+}
+__host__ __device__ inline double boundary_at(const kwgpu_option& o, const Boundary& bd, double s) {
+    if (bd.none || !(s > 0)) return 0.0;
+    double kk, x;
+    boundary_spot(o, bd, s, kk, x);
+    if (x <= boundary_bT(bd)) return boundary_exercised(bd, kk, x);
+    return kk * bd.t * boundary_integral(bd, x, 0, 1);
+}
+
+__host__ __device__ inline double boundary_premium(const kwgpu_option& o) {
+    return boundary_at(o, boundary_setup(o), o.s);
 }
 
 __host__ __device__ inline double premium_of(double y, double k, double base) {
@@ -304,7 +604,7 @@ __global__ void features_kernel(const kwgpu_option* opts, size_t n, GridArg g, c
     const bool in = option_features(o, g.x(i), g.t(i), f);
     for (int k = 0; k < F; ++k) feat[i * F + k] = f[k];
     if (target) {
-        const double base = baseline_premium(o);
+        const double base = boundary_premium(o);
         target[i] = (float)((premium[i] - base) / (base + kEps * o.k));
     }
     if (in_box) in_box[i] = in ? 1 : 0;
@@ -374,20 +674,43 @@ __global__ void node_features_kernel(const kwgpu_option* opts, size_t cn, GridAr
 // Ensemble combine in price units, as the FD pricer interpolates: member m's node values
 // v = BS + premium at x_{j-1} and x_j, its price ((x_j - x) v_{j-1} + (x - x_{j-1}) v_j) /
 // (x_j - x_{j-1}); premium[i] = mean price - BS at the spot, spread[i] = max - min price.
-__global__ void node_combine_kernel(const kwgpu_option* opts, size_t cn, GridArg g,
-                                    const float* y, int members, size_t ld, double* premium,
-                                    double* spread) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= cn) return;
+// Candidates' prices from their node values: kGroup lanes per candidate (launched with
+// kGroup * cn threads in 64-thread blocks). The group solves the candidate's exercise
+// boundary; lanes 0-7 then price the control variate at the first node and lanes 8-15
+// at the second, each lane summing every eighth quadrature point; lane 0 combines.
+__global__ __launch_bounds__(64) void node_combine_kernel(const kwgpu_option* opts, size_t cn,
+                                                          GridArg g, const float* y, int members,
+                                                          size_t ld, double* premium,
+                                                          double* spread) {
+    static_assert(kGroup == 16, "two nodes x 8 lanes");
+    const size_t gi = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t i = gi / kGroup;
+    const int j = (int)(gi % kGroup);
+    if (i >= cn) return;  // whole groups: cn counts groups
     const kwgpu_option o = opts[i];
     const NodePair q = node_pair(o, g.x(i), g.t(i), g.density, g.scale);
-    if (!q.ok) {
-        premium[i] = NAN;
-        spread[i] = INFINITY;
+    if (!q.ok) {  // the same for the whole group
+        if (j == 0) {
+            premium[i] = NAN;
+            spread[i] = INFINITY;
+        }
         return;
     }
-    const Baseline cr = baseline_setup(o);  // the nodes differ from the option only in spot
-    const double bwa = baseline_at(o, cr, q.a.s), bwb = baseline_at(o, cr, q.b.s);
+    const Boundary bd = boundary_setup_group(o, j);  // the nodes differ from o only in spot
+    double bw = 0.0;
+    if (!bd.none) {
+        double kk, x;
+        boundary_spot(o, bd, j < 8 ? q.a.s : q.b.s, kk, x);
+        const bool ex = x <= boundary_bT(bd);
+        double part = ex ? 0.0 : boundary_integral(bd, x, j & 7, 8);
+        part += __shfl_xor(part, 1, kGroup);
+        part += __shfl_xor(part, 2, kGroup);
+        part += __shfl_xor(part, 4, kGroup);
+        bw = ex ? boundary_exercised(bd, kk, x) : kk * bd.t * part;
+    }
+    const double bwb = __shfl(bw, 8, kGroup);
+    if (j != 0) return;
+    const double bwa = bw;
     const double bsa = euro_bs(q.a), bsb = euro_bs(q.b);
     const double ua = q.x1 - q.xs, ub = q.xs - q.x0, h = q.x1 - q.x0;
     double sum = 0, lo = 1e300, hi = -1e300;
@@ -1793,9 +2116,8 @@ int kwgpu_surrogate_predict(kwgpu_ctx* ctx, kwgpu_surrogate* s, const kwgpu_opti
         // 128-row tile and member
         hipLaunchKernelGGL(mlp_predict, dim3(nblk(2 * cn, kPredRows), s->members), dim3(256), 0,
                            st, s->x, s->p[0], s->wb[0], (int)(2 * cn), s->yall, (long)(2 * cn));
-        // one candidate per thread, each solving its baseline's critical spot: blocks of
-        // 64 spread a few thousand candidates over every CU
-        hipLaunchKernelGGL(node_combine_kernel, dim3(nblk(cn, 64)), dim3(64), 0, st,
+        // kGroup lanes per candidate, which together solve its exercise boundary
+        hipLaunchKernelGGL(node_combine_kernel, dim3(nblk(kGroup * cn, 64)), dim3(64), 0, st,
                            d_opts + c0, cn, g, s->yall, s->members, 2 * cn, d_premium + c0,
                            d_spread + c0);
         HIPCHK(hipGetLastError());

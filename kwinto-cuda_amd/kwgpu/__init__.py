@@ -476,8 +476,8 @@ def surrogate_labels(ctx: Context, d_opts: int, n: int, fd: Fd1dParams, d_nodes:
 
 
 def surrogate_baseline(ctx: Context, opts) -> np.ndarray:
-    """The analytic baseline premium (Ju & Zhong's correction of Barone-Adesi & Whaley)
-    the network's target is taken relative to, computed on the device."""
+    """The analytic baseline premium (Ju & Zhong's correction of Barone-Adesi & Whaley),
+    computed on the device: the gate's scale is this premium + 1e-4 K."""
     a = as_options(opts)
     out = np.full(len(a), np.nan)
     rc = lib().kwgpu_surrogate_baseline(ctx.h, a.ctypes.data, len(a), out.ctypes.data)

@@ -97,3 +97,94 @@ def _jz_chi(o, ss, a, w, x):
 
 def baseline_premium(o):
     return _premium(o, _jz_chi)
+
+
+# ---- the control variate: the premium from the exercise boundary ------------------------
+# The American put's exercise boundary b(tau) by Andersen, Lake & Offengenden's fixed point
+# (form "A"), restated step for step as the device computes it: 6 Chebyshev-Lobatto nodes
+# in sqrt(tau), 8 Jacobi steps from X (b_T / X)^{sqrt(tau / T)}, a 17-point tanh-sinh rule;
+# calls by put-call symmetry (swap S and K, r and q).
+TS_H = 0.375
+_TS_K = np.arange(-8, 9) * TS_H
+TS_A = 1.0 / (1.0 + np.exp(-np.pi * np.sinh(_TS_K)))            # (1 + tanh(pi/2 sinh kh)) / 2
+TS_W = 0.25 * np.pi * TS_H * np.cosh(_TS_K) / np.cosh(0.5 * np.pi * np.sinh(_TS_K)) ** 2
+BN, BIT = 6, 8
+
+
+def _ncdf(x):
+    return ndtr(x)
+
+
+def _cheb_fit(H):
+    i = np.arange(BN + 1)
+    c = np.cos(np.outer(i, i) * np.pi / BN)
+    w = np.ones(BN + 1)
+    w[0] = w[-1] = 0.5
+    a = (2.0 / BN) * (H * w) @ c.T
+    a[..., 0] *= 0.5
+    a[..., -1] *= 0.5
+    return a
+
+
+def _b_of(a, x0, st, xi):
+    """b at xi = sqrt(u): X exp(-sqrt(H)), H by Clenshaw over z = 1 - 2 xi / sqrt T."""
+    z = 1.0 - 2.0 * xi / st
+    b1 = np.zeros_like(z)
+    b2 = np.zeros_like(z)
+    for k in range(BN, 0, -1):
+        b1, b2 = 2 * z * b1 - b2 + a[..., k], b1
+    h = z * b1 - b2 + a[..., 0]
+    return x0 * np.exp(-np.sqrt(np.maximum(h, 0.0)))
+
+
+def boundary_premium(o):
+    """Early-exercise premium from the exercise boundary, per option (price units)."""
+    o = np.atleast_1d(o)
+    call = ~(o["w"] < 0)
+    r = np.where(call, o["q"], o["r"]).astype(float)
+    q = np.where(call, o["r"], o["q"]).astype(float)
+    s, T = o["z"].astype(float), o["t"].astype(float)
+    none = ~((o["k"] > 0) & (T > 0) & (s > 0)) | ~(r > 0)
+    out = np.zeros(len(o))
+    for n in np.nonzero(~none)[0]:
+        out[n] = _premium_one(float(r[n]), float(q[n]), float(s[n]), float(T[n]), bool(call[n]),
+                              float(o["k"][n]), float(o["s"][n]))
+    return out
+
+
+def _premium_one(r, q, s, T, call, K, S):
+    st = np.sqrt(T)
+    x0 = min(1.0, r / q) if q > 0 else 1.0
+    u = np.zeros(1, dtype=[("t", "<f8"), ("k", "<f8"), ("z", "<f8"), ("r", "<f8"), ("q", "<f8"),
+                           ("s", "<f8"), ("e", "u1"), ("w", "i1")])
+    u["t"], u["k"], u["z"], u["r"], u["q"], u["s"], u["e"], u["w"] = T, 1.0, s, r, q, 1.0, 1, -1
+    bT = min(float(critical_spot(u)[0][0]), x0)
+    f = 0.5 * (1.0 - np.cos(np.arange(BN + 1) * np.pi / BN))
+    tau = T * f * f
+    b = x0 * (bT / x0) ** f
+    b[0] = x0
+    for _ in range(BIT):
+        a = _cheb_fit(np.log(b / x0) ** 2)
+        ti = tau[1:, None]
+        u1, dt = ti * TS_A[None, :], ti * TS_A[::-1][None, :]
+        sd = s * np.sqrt(dt)
+        dm = (np.log(b[1:, None] / _b_of(a, x0, st, np.sqrt(u1))) + (r - q - 0.5 * s * s) * dt) / sd
+        ni = np.sum(TS_W * np.exp(r * u1) * _ncdf(dm), 1)
+        di = np.sum(TS_W * np.exp(q * u1) * _ncdf(dm + sd), 1)
+        sd0 = s * np.sqrt(tau[1:])
+        d0 = (np.log(b[1:]) + (r - q - 0.5 * s * s) * tau[1:]) / sd0
+        v = np.exp(-(r - q) * tau[1:]) * (_ncdf(d0) + r * tau[1:] * ni) / (_ncdf(d0 + sd0) + q * tau[1:] * di)
+        b[1:] = np.minimum(v, x0)
+    a = _cheb_fit(np.log(b / x0) ** 2)
+    kk = S if call else K
+    x = (K if call else S) / kk
+    if x <= _b_of(a, x0, st, np.array(st)):
+        sd = s * st
+        dm = (np.log(x) + (r - q - 0.5 * s * s) * T) / sd
+        eu = np.exp(-r * T) * _ncdf(-dm) - x * np.exp(-q * T) * _ncdf(-dm - sd)
+        return kk * ((1.0 - x) - eu)
+    u1, dt = T * TS_A, T * TS_A[::-1]
+    sd = s * np.sqrt(dt)
+    dm = (np.log(x / _b_of(a, x0, st, np.sqrt(u1))) + (r - q - 0.5 * s * s) * dt) / sd
+    p = np.sum(TS_W * (r * np.exp(-r * dt) * _ncdf(-dm) - q * x * np.exp(-q * dt) * _ncdf(-dm - sd)))
+    return kk * T * p

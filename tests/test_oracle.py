@@ -305,6 +305,44 @@ def test_baw_baseline_tracks_the_fd_american_price():
     assert np.all(baw_premium(z) == 0)
 
 
+def test_boundary_premium_is_the_fd_solvers_limit():
+    """The surrogate's control variate, the premium from the exercise boundary's fixed
+    point, against the FD American price as the grid is refined: the gap halves with
+    each doubling of the grid (the FD scheme converges at first order under the
+    early-exercise projection), so the boundary premium is the limit the FD prices
+    approach. A wrong term in the boundary equation or the premium integral leaves a
+    gap that does not shrink. It also pays nothing where early exercise never pays,
+    prices calls as the put with S and K, r and q exchanged, and pays the exercise
+    value minus the European price below the critical spot."""
+    from baw_ref import baseline_premium, boundary_premium, european
+    o = W.surrogate_train(96, seed=11)
+    o["s"] = o["k"] * np.exp(np.random.default_rng(3).uniform(-0.3, 0.3, len(o)))
+    bp = boundary_premium(o)
+    eu, _ = european(o)
+    rms = []
+    for nx in (512, 1024, 2048):
+        fd, err = oracle.fd1d_price(o, nx, nx)
+        assert err == ""
+        rms.append(float(np.sqrt(np.mean(((fd - eu) - bp) ** 2))))
+    # measured: 5.1e-3, 2.4e-3, 1.25e-3
+    assert rms[1] < 0.6 * rms[0] and rms[2] < 0.6 * rms[1], rms
+    fd, _ = oracle.fd1d_price(o, 1024, 1024)
+    assert rms[1] < 0.1 * float(np.sqrt(np.mean(((fd - eu) - baseline_premium(o)) ** 2)))
+    assert np.all(bp >= -1e-12)
+    z = o.copy()
+    z["q"] = np.where(z["w"] > 0, 0.0, z["q"])
+    z["r"] = np.where(z["w"] < 0, 0.0, z["r"])
+    assert np.all(boundary_premium(z) == 0)
+    c = o[o["w"] > 0][:8].copy()
+    p = c.copy()
+    p["w"], p["s"], p["k"], p["r"], p["q"] = -1, c["k"], c["s"], c["q"], c["r"]
+    assert np.allclose(boundary_premium(c), boundary_premium(p), rtol=1e-12, atol=1e-12)
+    deep = o[(o["w"] < 0) & (o["r"] > o["q"] + 0.01)][:8].copy()  # exercise region: b(T)
+    deep["s"] = deep["k"] * 0.05                                    # is near K when r > q
+    ed, _ = european(deep)
+    assert np.allclose(boundary_premium(deep), (deep["k"] - deep["s"]) - ed, rtol=0, atol=1e-12)
+
+
 def test_ju_zhong_baseline_improves_on_baw():
     """The baseline the surrogate uses, Ju & Zhong's correction of BAW, against the FD
     American price over the training box's maturities (to 5 years, where BAW's error is

@@ -263,10 +263,11 @@ def test_training_learns_the_premium(ctx):
 def test_node_form_matches_fd_interpolation(ctx):
     """The surrogate forms a price from its two node values as the FD pricer does
     (src/Math/kwFd1d.cpp:139-158). With the network's weights zeroed except the output
-    bias, the premium at every node is base + (base + 1e-4 K) b5 (base: the analytic
-    baseline), so the price is BS plus that, interpolated with the reference's weights
-    at the bracketing nodes (computed here from the sinh mesh) -- to rounding."""
-    from baw_ref import baseline_premium
+    bias, the premium at every node is base + (base + 1e-4 K) b5 (base: the control
+    variate, the premium from the exercise boundary), so the price is BS plus that,
+    interpolated with the reference's weights at the bracketing nodes (computed here
+    from the sinh mesh) -- to rounding."""
+    from baw_ref import boundary_premium
     from node_ref import DAMP_MIN, bracket, kink_damping, nodes
     o = W.surrogate_train(2000, 77)
     fd = kwgpu.params(512, 256)
@@ -296,13 +297,14 @@ def test_node_form_matches_fd_interpolation(ctx):
     ea, eb, es = a.copy(), b.copy(), o.copy()
     for q in (ea, eb, es):
         q["e"] = 0
-    va = ctx.bs_price(ea) + baseline_premium(a) + (baseline_premium(a) + 1e-4 * o["k"]) * b5
-    vb = ctx.bs_price(eb) + baseline_premium(b) + (baseline_premium(b) + 1e-4 * o["k"]) * b5
+    pa, pb = boundary_premium(a), boundary_premium(b)
+    va = ctx.bs_price(ea) + pa + (pa + 1e-4 * o["k"]) * b5
+    vb = ctx.bs_price(eb) + pb + (pb + 1e-4 * o["k"]) * b5
     want = ((x1 - xs) * va + (xs - x0) * vb) / (x1 - x0) - ctx.bs_price(es)
     d = np.abs(pm[ok] - want[ok])
     _say("  node form vs restatement: max", d.max())
-    # the device and numpy baselines agree to ~1e-9 K (test_baseline_matches_numpy_...),
-    # scaled here by 1 + b5; a wrong weight or node would be off by 1e-4 to 1e-2
+    # the device's control variate and its restatement agree to rounding, scaled here by
+    # 1 + b5; a wrong weight or node would be off by 1e-4 to 1e-2
     assert d.max() <= 2e-9 * (1 + b5) * o["k"].max()
     s.close()
 
@@ -325,7 +327,7 @@ def test_prediction_matches_forward_oracle(ctx):
     dropped bias moves the network's share of the premium by far more than 3e-3 of
     itself. FP32 accumulation against float64 decides some BF16 rounding ties the
     other way, and those ties compound over three layers: measured up to 8e-4."""
-    from baw_ref import baseline_premium
+    from baw_ref import boundary_premium
     from node_ref import bracket, nodes
     o = W.surrogate_train(1500, 79)
     fd = kwgpu.params(512, 256)
@@ -355,7 +357,7 @@ def test_prediction_matches_forward_oracle(ctx):
     ea, eb, es = a.copy(), b.copy(), o.copy()
     for q in (ea, eb, es):
         q["e"] = 0
-    ba, bb = baseline_premium(a), baseline_premium(b)
+    ba, bb = boundary_premium(a), boundary_premium(b)
     sa, sb = ba + 1e-4 * o["k"], bb + 1e-4 * o["k"]
     fa, fb = kwgpu.surrogate_features(a, 512, 256), kwgpu.surrogate_features(b, 512, 256)
     v, net = [], np.zeros(len(o))

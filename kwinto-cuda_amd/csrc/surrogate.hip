@@ -218,8 +218,10 @@ __host__ __device__ inline double baseline_premium(const kwgpu_option& o) {
 // error, since the same at 16 nodes and 16 steps changes it by under 1e-4 -- against
 // 0.064 for Ju & Zhong. No early exercise (0): puts with r <= 0, calls with q <= 0.
 constexpr int kBn = 6, kBit = 8, kTs = 17;
-// tanh-sinh abscissae a_k = (1 + tanh(pi/2 sinh(k h))) / 2 and weights, h = 0.375, |k| <= 8
-// (a_{16-k} = 1 - a_k to the last digit, so tau - u is taken from the mirror point)
+// tanh-sinh rule on (0, 1), h = 0.375, |k| <= 8: abscissae a_k = (1 + tanh(pi/2 sinh(kh))) / 2
+// and weights w_k = (h pi/4) cosh(kh) / cosh^2(pi/2 sinh(kh)), so that
+// int_0^tau f(u) du ~ tau sum_k w_k f(tau a_k) (the w_k sum to 1 + 8e-9). a_{16-k} = 1 - a_k
+// to the last digit, so tau - u is taken from the mirror point.
 __host__ __device__ inline double ts_a(int k) {
     constexpr double a[kTs] = {
         2.1470805279391156e-14, 4.2864745391083903e-10, 3.9763144264366805e-07,

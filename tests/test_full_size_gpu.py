@@ -233,5 +233,7 @@ def test_serving_1m_full_size(ctx, fitted):
                                         d_r.data_ptr()))
     _say(f"  plain FP64 FD {t_fd * 1e3:.1f} ms, surrogate-first {t_sf * 1e3:.1f} ms: "
          f"{t_fd / t_sf:.3f}x with {served.mean():.3f} of the batch served")
-    assert served.mean() >= 0.04
-    assert t_fd / t_sf >= 1.02
+    # measured: 31 % served, 1.36x (the served share is set by the fit, which is the
+    # same on every box; the ratio by both solves on the same box)
+    assert served.mean() >= 0.2
+    assert t_fd / t_sf >= 1.15

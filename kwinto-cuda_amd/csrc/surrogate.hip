@@ -1465,8 +1465,10 @@ struct kwgpu_surrogate {
     float *x = nullptr, *tg = nullptr, *y = nullptr, *dy = nullptr, *Z = nullptr;
     bf16 *Hb = nullptr, *dZa = nullptr, *dZb = nullptr;
     uint32_t* idx = nullptr;
-    float* yall = nullptr;
+    float* yall = nullptr;  // prediction: the members' outputs, 2 rows per option
     size_t yall_cap = 0;
+    float* xp = nullptr;    // prediction: node features, 2 rows per option
+    size_t xp_cap = 0;
     float* part = nullptr;  // per-block partial sums of the column reductions (kPartRows
                             // blocks x kPartWidth per member)
     void* serve = nullptr;  // kwgpu_surrogate_price's per-option buffers (grown on demand)
@@ -1749,7 +1751,7 @@ void kwgpu_surrogate_destroy(kwgpu_surrogate* s) {
     free_scratch(s);
     for (void* q : {(void*)s->pall, (void*)s->wball, (void*)s->wballT, (void*)s->grad,
                     (void*)s->m1, (void*)s->m2,
-                    (void*)s->loss, (void*)s->yall, (void*)s->part, s->serve})
+                    (void*)s->loss, (void*)s->yall, (void*)s->xp, (void*)s->part, s->serve})
         if (q) (void)hipFree(q);
     if (s->ev_done) (void)hipEventDestroy(s->ev_done);
     delete s;
@@ -2096,13 +2098,24 @@ int kwgpu_surrogate_predict(kwgpu_ctx* ctx, kwgpu_surrogate* s, const kwgpu_opti
     if (n == 0) return KWGPU_OK;
     hipStream_t st = stream ? (hipStream_t)stream : kwgpu::ctx_stream(ctx);
     if (s->done_stream && s->done_stream != st) HIPCHK(hipStreamWaitEvent(st, s->ev_done, 0));
-    // two rows per option (the bracketing nodes); scratch for 2 CH rows per member
-    constexpr size_t CH = 16384;
-    HIPCHK(ensure_scratch(s, 2 * std::min(n, CH)));
-    const size_t need = (size_t)s->members * 2 * std::min(n, CH);
+    // two rows per option (the bracketing nodes), in chunks of up to CH options: the
+    // features and the members' outputs of one chunk (64 + 32 bytes per option for 4
+    // members). A chunk is large so that a batch is seldom split: a small last chunk
+    // would pay the full latency of its boundary solves and network launch.
+    constexpr size_t CH = size_t(1) << 20;
+    const size_t rows = 2 * std::min(n, CH);
+    if (s->xp_cap < rows) {
+        if (s->xp) HIPCHK(hipFree(s->xp));
+        s->xp = nullptr;
+        s->xp_cap = 0;
+        HIPCHK(hipMalloc(&s->xp, rows * F * sizeof(float)));
+        s->xp_cap = rows;
+    }
+    const size_t need = (size_t)s->members * rows;
     if (s->yall_cap < need) {
         if (s->yall) HIPCHK(hipFree(s->yall));
         s->yall = nullptr;
+        s->yall_cap = 0;
         HIPCHK(hipMalloc(&s->yall, need * sizeof(float)));
         s->yall_cap = need;
     }
@@ -2113,11 +2126,11 @@ int kwgpu_surrogate_predict(kwgpu_ctx* ctx, kwgpu_surrogate* s, const kwgpu_opti
         if (g.x_dims) g.x_dims += c0;
         if (g.t_dims) g.t_dims += c0;
         hipLaunchKernelGGL(node_features_kernel, dim3(nblk(cn, 256)), dim3(256), 0, st,
-                           d_opts + c0, cn, g, s->x, d_in_box ? d_in_box + c0 : nullptr);
+                           d_opts + c0, cn, g, s->xp, d_in_box ? d_in_box + c0 : nullptr);
         // the whole ensemble at once on the shared features, one fused kernel per
         // 128-row tile and member
         hipLaunchKernelGGL(mlp_predict, dim3(nblk(2 * cn, kPredRows), s->members), dim3(256), 0,
-                           st, s->x, s->p[0], s->wb[0], (int)(2 * cn), s->yall, (long)(2 * cn));
+                           st, s->xp, s->p[0], s->wb[0], (int)(2 * cn), s->yall, (long)(2 * cn));
         // kGroup lanes per candidate, which together solve its exercise boundary
         hipLaunchKernelGGL(node_combine_kernel, dim3(nblk(kGroup * cn, 64)), dim3(64), 0, st,
                            d_opts + c0, cn, g, s->yall, s->members, 2 * cn, d_premium + c0,

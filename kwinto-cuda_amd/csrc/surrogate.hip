@@ -1356,11 +1356,12 @@ __global__ __launch_bounds__(kGateBlock) void compact_routes(const int32_t* rout
 }
 
 // Serving candidates: in the training box, with a labelled bracket, in a class the gate
-// leaves open, and with a baseline scale base + 1e-4 K (the price scale of the network's
-// error) of at most that class's scale bound; flag[i] = 1 for a candidate, else 0.
-// Launched in blocks of kCandBlock: an option's work (the baseline's critical spot) is
-// long and a batch can be a few ten thousand options, so small blocks spread it over
-// every CU (in 256-thread blocks a 32768-option batch left half the CUs idle).
+// leaves open, and with a baseline scale (the Ju & Zhong premium + 1e-4 K, about the
+// price scale of the network's error) of at most that class's scale bound; flag[i] = 1
+// for a candidate, else 0. Launched in blocks of kCandBlock: an option's work (the
+// critical spot) is long and a batch can be a few ten thousand options, so small blocks
+// spread it over every CU (in 256-thread blocks a 32768-option batch left half the CUs
+// idle).
 constexpr int kCandBlock = 64;
 
 __global__ __launch_bounds__(kCandBlock) void candidate_kernel(const kwgpu_option* opts, size_t n,

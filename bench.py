@@ -68,6 +68,8 @@ def parse_args():
                     help="load the 1024 x 1024 surrogate from this file (saved by --sur-save "
                          "from this configuration) instead of fitting it; it is evaluated and "
                          "its gate calibrated as after a fit")
+    ap.add_argument("--sur-po-calibrate", type=int, default=16 << 20,
+                    help="calibration options of the per-option-grid gate")
     ap.add_argument("--sur-per-option", type=int, default=1,
                     help="also fit a surrogate to the per-option-grid FD pricer and serve "
                          "the batch on per-option grids")
@@ -389,9 +391,12 @@ def bench_surrogate(ctx, args, log, head=None):
     sur.close()
     uneven = {}
     if args.sur_per_option:
+        # the per-option gate rests on a 16M calibration set: its errors have a longer
+        # tail than the one-grid ensemble's, and a 4M set once let 1.07e-4 through
         sur2, train2 = pipeline.train_and_evaluate(ctx, n_train=args.sur_train - n_test,
                                                    n_test=n_test, epochs=args.sur_epochs,
-                                                   per_option_grids=True, log=log)
+                                                   per_option_grids=True,
+                                                   n_calibrate=args.sur_po_calibrate, log=log)
         xd, td = W.serving_grids(opts, 5)
         uneven["training"] = train2
         for name, prec in (("fp64", kwgpu.FP64), ("fp32_mixed", kwgpu.FP32_MIXED)):
